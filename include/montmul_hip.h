@@ -1,0 +1,200 @@
+/*
+ * montmul_hip.h - C ABI of the MI355X (gfx950) backend for batched Montgomery squaring.
+ *
+ * Two layers, both plain C types, no exceptions across the boundary:
+ *
+ *  1. The backend (one batch of slots on one device). These entry points are what a
+ *     foreign-language `impl Backend for Hip` binds; each replaces one method of the
+ *     reference's `Backend`/`Limits` traits (montmul/src/transform.rs:11-20, 61-77), whose
+ *     GPU implementation is `Gpu` (montmul/src/gpu/mod.rs:272-398).
+ *
+ *  2. The host API (`ModBatch`, `fermat`, `Params`; montmul/src/batch.rs:9-198), written in
+ *     C++ over any backend given as an `mm_backend` table, so that the same host code drives
+ *     this library's `Hip` backend or any other implementation of the table.
+ *
+ * Conventions
+ *  - Status codes: 0 = success, nonzero = error; the message of the calling thread's last
+ *    error is returned by mm_last_error(). `run` and `mul` return MM_OK (-1) on success, the
+ *    index (>= 0) of the first slot whose carry check failed (the reference panics with
+ *    "... slot i", gpu/mod.rs:179-184), or MM_EFAULT (-2) on any other failure.
+ *  - Every call that touches the device makes the batch's device current on the calling
+ *    thread first (HIP's current device is per thread).
+ *  - Limbs are 32-bit little-endian: limb q holds bits 32q .. 32q+31.
+ *  - Exponents are most significant bit first: bit p is (words[p / 32] >> (31 - p % 32)) & 1
+ *    (transform.rs:67).
+ */
+#ifndef MONTMUL_HIP_H
+#define MONTMUL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_OK ((int64_t)-1)
+#define MM_EFAULT ((int64_t)-2)
+
+/* Static data of one batch. Replaces `BatchSetup` (transform.rs:49-59). */
+typedef struct mm_batch_setup {
+    uint32_t k;              /* limbs per residue */
+    uint32_t len;            /* transform length L (real coefficients; L/2 complex elements) */
+    uint32_t base;           /* c, the multiplier applied on exponent bits equal to 1 */
+    uint32_t slots;          /* number of moduli */
+    const uint32_t *n;       /* slots * k limbs: N */
+    const uint32_t *nprime;  /* slots * k limbs: N' = -N^-1 mod R, R = 2^(32k) */
+    const uint32_t *two_n;   /* slots * k limbs: 2N */
+} mm_batch_setup;
+
+/*
+ * A backend as a table of functions over opaque state. Replaces the generic parameter
+ * `B: Backend + Limits` of `ModBatch<B>` (batch.rs:53-58, transform.rs:11-20, 61-77).
+ */
+typedef struct mm_backend {
+    void *self;
+    /* Backend::create (transform.rs:64). Writes an opaque state to *state. */
+    int (*create)(void *self, const mm_batch_setup *setup, void **state);
+    void (*destroy)(void *state);
+    /* Backend::load (transform.rs:66): k limbs of a Montgomery-form value in [0, 2N). */
+    int (*load)(void *state, size_t slot, const uint32_t *limbs);
+    /* Backend::set_exponent (transform.rs:68); resets the slot's position to 0. */
+    int (*set_exponent)(void *state, size_t slot, const uint32_t *words, size_t nwords, uint64_t nbits);
+    /* Backend::run (transform.rs:70): up to `bits` steps on every slot with bits left. */
+    int64_t (*run)(void *state, uint64_t bits);
+    /* Backend::mul (transform.rs:72): slot out <- a * b * R^-1 (shared modulus). */
+    int64_t (*mul)(void *state, size_t out, size_t a, size_t b);
+    /* Backend::read (transform.rs:74): k limbs of slot `slot`. */
+    int (*read)(void *state, size_t slot, uint32_t *out);
+    /* Backend::remaining (transform.rs:76). */
+    uint64_t (*remaining)(void *state, size_t slot);
+    /* Message of the calling thread's last error of this backend. */
+    const char *(*last_error)(void);
+    /* Limits::coefficient_bound (transform.rs:13), as a 128-bit value. */
+    uint64_t bound_lo, bound_hi;
+    /* Limits::supported_lengths (transform.rs:15): every power of two in [min_len, max_len]. */
+    uint32_t min_len, max_len;
+    /* Limits::full_k (transform.rs:17): batches use k = L/2. */
+    int32_t full_k;
+} mm_backend;
+
+/* ------------------------------------------------------------------------------------ */
+/* Layer 1: the Hip backend.                                                             */
+/* ------------------------------------------------------------------------------------ */
+
+/* Replaces `GpuOptions` (gpu/mod.rs:28-45). */
+typedef struct mm_options {
+    int32_t device;          /* HIP device ordinal */
+    int32_t force_global;    /* keep transform vectors in HBM even when LDS would hold them */
+    int32_t wraparound;      /* reduction through the half-length cyclic product; k = L/2 */
+    uint32_t max_launch_ms;  /* target duration of one kernel launch; 0 = no splitting */
+} mm_options;
+
+/* Launch plan of a batch. Replaces `LaunchPlan`/`Mode` (gpu/plan.rs:5-22). */
+typedef struct mm_plan_info {
+    int32_t global;          /* 0: vector resident in LDS; 1: vector in HBM, LDS tiles */
+    uint32_t threads;        /* threads per workgroup */
+    uint32_t lds_bytes;      /* dynamic LDS per workgroup */
+    uint32_t slots_per_wg;   /* slots handled by one workgroup */
+} mm_plan_info;
+
+/* Replaces Gpu::device_count (gpu/mod.rs:70-72). Returns -1 on error. */
+int mm_device_count(void);
+/* Replaces Gpu::with_options (gpu/mod.rs:62-67): fills `out` with this backend's table. */
+int mm_hip_open(const mm_options *opts, mm_backend *out);
+/* Releases what mm_hip_open allocated; every state created from it must be destroyed first. */
+void mm_hip_close(mm_backend *be);
+
+/* The table's functions, also callable directly with the `self`/state of a Hip table. */
+int mm_create(void *self, const mm_batch_setup *setup, void **state);    /* gpu/mod.rs:325-370 */
+void mm_destroy(void *state);                                          /* state drop */
+int mm_load(void *state, size_t slot, const uint32_t *limbs);          /* gpu/mod.rs:371-375 */
+int mm_set_exponent(void *state, size_t slot, const uint32_t *words, size_t nwords,
+                    uint64_t nbits);                                   /* gpu/mod.rs:376-382 */
+int64_t mm_run(void *state, uint64_t bits);                            /* gpu/mod.rs:383-385 */
+int64_t mm_mul(void *state, size_t out, size_t a, size_t b);           /* gpu/mod.rs:386-391 */
+int mm_read(void *state, size_t slot, uint32_t *out);                  /* gpu/mod.rs:392-394 */
+uint64_t mm_remaining(void *state, size_t slot);                       /* gpu/mod.rs:395-397 */
+const char *mm_last_error(void);
+
+/* GPU-only extras (gpu/mod.rs:79-111). */
+/* Gpu::run_bare (gpu/mod.rs:109-111): each step is the squaring alone; x is left unchanged. */
+int64_t mm_run_bare(void *state, uint64_t bits);
+/* Uploads loaded values and exponents now instead of at the next run (gpu/mod.rs:161-176). */
+int mm_push(void *state);
+/* Gpu::plan_of (gpu/mod.rs:102-104). */
+int mm_plan(void *state, mm_plan_info *out);
+/* Gpu::free_memory, sm_count, max_dyn_smem, device_name (gpu/mod.rs:83-99). */
+uint64_t mm_free_memory(void *self);
+int mm_cu_count(void *self);
+uint64_t mm_max_lds(void *self);
+int mm_device_name(void *self, char *buf, size_t len);
+/* Device time of the step kernel launches of the latest run, in milliseconds (HIP events on
+ * the batch's stream), and the number of launches. No reference counterpart: measurement. */
+int mm_last_run_kernel_ms(void *state, double *ms, uint32_t *launches);
+
+/* ------------------------------------------------------------------------------------ */
+/* Layer 2: the host API over any backend table.                                          */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct mm_modbatch mm_modbatch;
+
+/* Params::for_bits (batch.rs:20-25): k = ceil((bits + 4) / 32), L = the smallest supported
+ * length >= 2k. Returns nonzero when no supported length is large enough. */
+int mm_params_for_bits(uint32_t bits, uint32_t min_len, uint32_t max_len, uint32_t *k, uint32_t *len);
+/* Params::check_exact (batch.rs:27-31): max(c, 1) * k * (2^32 - 1)^2 < bound. */
+int mm_params_check_exact(uint64_t k, uint32_t c, uint64_t bound_lo, uint64_t bound_hi);
+
+/*
+ * ModBatch::new (batch.rs:62-105). Moduli are given as `count` signed integers: modulus i
+ * has limbs limbs[offsets[i] .. offsets[i+1]) (little-endian magnitude) and sign signs[i]
+ * (-1, 0 or 1). The backend table must outlive the batch.
+ */
+int mm_modbatch_new(const mm_backend *be, size_t count, const uint32_t *limbs, const size_t *offsets,
+                    const int8_t *signs, uint32_t base, mm_modbatch **out);
+void mm_modbatch_free(mm_modbatch *b);
+/* ModBatch::params (batch.rs:107-109) and slots (110-112). */
+void mm_modbatch_params(const mm_modbatch *b, uint32_t *k, uint32_t *len);
+size_t mm_modbatch_slots(const mm_modbatch *b);
+/* The backend state, for backend-specific extras (ModBatch::backend_and_state, batch.rs:120-122). */
+void *mm_modbatch_state(mm_modbatch *b);
+/* ModBatch::load (batch.rs:125-130): any integer x, stored as x * R mod N. */
+int mm_modbatch_load(mm_modbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
+/* ModBatch::load_raw (batch.rs:133-137): fails unless 0 <= xm < 2N. */
+int mm_modbatch_load_raw(mm_modbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
+/* ModBatch::set_exponent (batch.rs:140-149); e >= 0. */
+int mm_modbatch_set_exponent(mm_modbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
+/* ModBatch::run (batch.rs:152-154). Returns as mm_run. */
+int64_t mm_modbatch_run(mm_modbatch *b, uint64_t bits);
+/* ModBatch::remaining (batch.rs:157-159). */
+uint64_t mm_modbatch_remaining(mm_modbatch *b, size_t slot);
+/* ModBatch::mul (batch.rs:162-166); slots must share a modulus. Returns as mm_mul. */
+int64_t mm_modbatch_mul(mm_modbatch *b, size_t out, size_t a, size_t c);
+/* ModBatch::read_raw (batch.rs:168-170): k limbs in [0, 2N). */
+int mm_modbatch_read_raw(mm_modbatch *b, size_t slot, uint32_t *out);
+/* ModBatch::read (batch.rs:173-176): k limbs of raw * R^-1 mod N, in [0, N). */
+int mm_modbatch_read(mm_modbatch *b, size_t slot, uint32_t *out);
+/* ModBatch::is_one (batch.rs:179-183): 1 if the raw value is R mod N or R mod N + N, 0 if
+ * not, negative on error. */
+int mm_modbatch_is_one(mm_modbatch *b, size_t slot);
+/* k limbs of slot i's modulus (ModBatch::modulus, batch.rs:113-115). */
+int mm_modbatch_modulus(mm_modbatch *b, size_t slot, uint32_t *out);
+
+/* fermat (batch.rs:187-198): base-`base` Fermat PRP test of every modulus; prp[i] = 1 or 0. */
+int mm_fermat(const mm_backend *be, size_t count, const uint32_t *limbs, const size_t *offsets,
+              const int8_t *signs, uint32_t base, uint64_t chunk, uint8_t *prp);
+
+/* The score workload (montmul/src/score.rs:45-61): `slots` moduli of `bits` bits drawn from
+ * GMP's default random state seeded with `seed`, top bit and bit 0 set, and per modulus the top
+ * `ebits` bits of N - 1. Writes ceil(bits/32) limbs per modulus and ceil(ebits/32) per exponent. */
+int mm_score_workload(uint32_t bits, size_t slots, uint32_t ebits, uint64_t seed, uint32_t *moduli,
+                      uint32_t *exps);
+
+/* Message of the calling thread's last host-layer error. */
+const char *mm_host_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MONTMUL_HIP_H */

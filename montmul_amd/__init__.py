@@ -1,0 +1,409 @@
+"""montmul_amd: batched Montgomery squaring for many odd moduli on MI355X (gfx950).
+
+A thin Python layer over the C ABI of libmontmul_hip.so (include/montmul_hip.h). It mirrors
+the reference's host API (montmul/src/batch.rs, montmul/src/transform.rs) with Python
+integers in place of `rug::Integer`:
+
+    ModBatch(backend, moduli, base)      ModBatch::new          (batch.rs:62-105)
+      .load / .load_raw / .set_exponent  (batch.rs:125-149)
+      .run / .remaining / .mul           (batch.rs:152-166)
+      .read_raw / .read / .is_one        (batch.rs:168-183)
+    fermat(backend, moduli, base, chunk) (batch.rs:187-198)
+    Params.for_bits / Params.check_exact (batch.rs:20-31)
+    Hip(...)                             the GPU backend (replaces `Gpu`, gpu/mod.rs)
+
+Errors that the reference reports with `Err` or a panic raise MontmulError; a carry out of a
+product that must fit raises CarryError naming the slot ("... slot i", gpu/mod.rs:179-184).
+The library must be built (python -m montmul_amd.build); there is no fallback path.
+"""
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmontmul_hip.so")
+
+__all__ = ["MontmulError", "CarryError", "Backend", "Hip", "ModBatch", "Params", "fermat",
+           "score_workload", "lib", "LIB_PATH", "MM_OK", "MM_EFAULT"]
+
+MM_OK = -1
+MM_EFAULT = -2
+
+
+class MontmulError(RuntimeError):
+    pass
+
+
+class CarryError(MontmulError):
+    """A carry out of a product that must fit; `slot` names the first such slot."""
+
+    def __init__(self, msg, slot):
+        super().__init__(msg)
+        self.slot = slot
+
+
+class BatchSetup(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("len", C.c_uint32), ("base", C.c_uint32), ("slots", C.c_uint32),
+                ("n", C.POINTER(C.c_uint32)), ("nprime", C.POINTER(C.c_uint32)),
+                ("two_n", C.POINTER(C.c_uint32))]
+
+
+_CREATE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(BatchSetup), C.POINTER(C.c_void_p))
+_DESTROY = C.CFUNCTYPE(None, C.c_void_p)
+_LOAD = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32))
+_SETEXP = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t, C.c_uint64)
+_RUN = C.CFUNCTYPE(C.c_int64, C.c_void_p, C.c_uint64)
+_MUL = C.CFUNCTYPE(C.c_int64, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t)
+_READ = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32))
+_REMAINING = C.CFUNCTYPE(C.c_uint64, C.c_void_p, C.c_size_t)
+_LASTERR = C.CFUNCTYPE(C.c_char_p)
+
+
+class BackendTable(C.Structure):
+    """`mm_backend`: the backend as a table of functions (transform.rs:11-20, 61-77)."""
+    _fields_ = [("self", C.c_void_p), ("create", _CREATE), ("destroy", _DESTROY), ("load", _LOAD),
+                ("set_exponent", _SETEXP), ("run", _RUN), ("mul", _MUL), ("read", _READ),
+                ("remaining", _REMAINING), ("last_error", _LASTERR),
+                ("bound_lo", C.c_uint64), ("bound_hi", C.c_uint64),
+                ("min_len", C.c_uint32), ("max_len", C.c_uint32), ("full_k", C.c_int32)]
+
+
+class Options(C.Structure):
+    _fields_ = [("device", C.c_int32), ("force_global", C.c_int32), ("wraparound", C.c_int32),
+                ("max_launch_ms", C.c_uint32)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("global_", C.c_int32), ("threads", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("slots_per_wg", C.c_uint32)]
+
+
+def _load_lib():
+    if not os.path.exists(LIB_PATH):
+        raise MontmulError(f"{LIB_PATH} is not built; run `python -m montmul_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    u32p, vp = C.POINTER(C.c_uint32), C.c_void_p
+    sig = {
+        "mm_device_count": (C.c_int, []),
+        "mm_hip_open": (C.c_int, [C.POINTER(Options), C.POINTER(BackendTable)]),
+        "mm_hip_close": (None, [C.POINTER(BackendTable)]),
+        "mm_last_error": (C.c_char_p, []),
+        "mm_host_last_error": (C.c_char_p, []),
+        "mm_run_bare": (C.c_int64, [vp, C.c_uint64]),
+        "mm_push": (C.c_int, [vp]),
+        "mm_plan": (C.c_int, [vp, C.POINTER(PlanInfo)]),
+        "mm_free_memory": (C.c_uint64, [vp]),
+        "mm_cu_count": (C.c_int, [vp]),
+        "mm_max_lds": (C.c_uint64, [vp]),
+        "mm_device_name": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
+        "mm_last_run_kernel_ms": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+        "mm_params_for_bits": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p]),
+        "mm_params_check_exact": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]),
+        "mm_modbatch_new": (C.c_int, [C.POINTER(BackendTable), C.c_size_t, u32p, C.POINTER(C.c_size_t),
+                                      C.POINTER(C.c_int8), C.c_uint32, C.POINTER(vp)]),
+        "mm_modbatch_free": (None, [vp]),
+        "mm_modbatch_params": (None, [vp, u32p, u32p]),
+        "mm_modbatch_slots": (C.c_size_t, [vp]),
+        "mm_modbatch_state": (vp, [vp]),
+        "mm_modbatch_load": (C.c_int, [vp, C.c_size_t, C.c_int, u32p, C.c_size_t]),
+        "mm_modbatch_load_raw": (C.c_int, [vp, C.c_size_t, C.c_int, u32p, C.c_size_t]),
+        "mm_modbatch_set_exponent": (C.c_int, [vp, C.c_size_t, C.c_int, u32p, C.c_size_t]),
+        "mm_modbatch_run": (C.c_int64, [vp, C.c_uint64]),
+        "mm_modbatch_remaining": (C.c_uint64, [vp, C.c_size_t]),
+        "mm_modbatch_mul": (C.c_int64, [vp, C.c_size_t, C.c_size_t, C.c_size_t]),
+        "mm_modbatch_read_raw": (C.c_int, [vp, C.c_size_t, u32p]),
+        "mm_modbatch_read": (C.c_int, [vp, C.c_size_t, u32p]),
+        "mm_modbatch_is_one": (C.c_int, [vp, C.c_size_t]),
+        "mm_modbatch_modulus": (C.c_int, [vp, C.c_size_t, u32p]),
+        "mm_fermat": (C.c_int, [C.POINTER(BackendTable), C.c_size_t, u32p, C.POINTER(C.c_size_t),
+                                C.POINTER(C.c_int8), C.c_uint32, C.c_uint64, C.POINTER(C.c_uint8)]),
+        "mm_score_workload": (C.c_int, [C.c_uint32, C.c_size_t, C.c_uint32, C.c_uint64, u32p, u32p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+class _Lib:
+    """The loaded library, opened on first use (so that `montmul_amd.build` can run first)."""
+
+    _handle = None
+
+    def __getattr__(self, name):
+        if _Lib._handle is None:
+            _Lib._handle = _load_lib()
+        return getattr(_Lib._handle, name)
+
+
+lib = _Lib()
+
+
+def _host_error():
+    return (lib.mm_host_last_error() or b"").decode(errors="replace")
+
+
+# ---- integers <-> limbs --------------------------------------------------------------------
+
+def to_limbs(x, k=None):
+    """Little-endian 32-bit limbs of |x| (k limbs when given; at least one)."""
+    mag = abs(x)
+    nl = max(1, (mag.bit_length() + 31) // 32) if k is None else k
+    if mag.bit_length() > 32 * nl:
+        raise MontmulError(f"value does not fit {nl} limbs")
+    return (C.c_uint32 * nl).from_buffer_copy(mag.to_bytes(4 * nl, "little"))
+
+
+def from_limbs(buf):
+    return int.from_bytes(bytes(buf), "little")
+
+
+def _sign(x):
+    return (x > 0) - (x < 0)
+
+
+def _pack_ints(values):
+    """Concatenated limbs, offsets and signs of several integers (one C call each side)."""
+    chunks, offsets, signs = [], [0], []
+    for v in values:
+        mag = abs(int(v))
+        nl = max(1, (mag.bit_length() + 31) // 32)
+        chunks.append(mag.to_bytes(4 * nl, "little"))
+        offsets.append(offsets[-1] + nl)
+        signs.append(_sign(int(v)))
+    data = b"".join(chunks)
+    limbs = (C.c_uint32 * max(1, offsets[-1])).from_buffer_copy(data + b"\0" * (4 * max(1, offsets[-1]) - len(data)))
+    return limbs, (C.c_size_t * len(offsets))(*offsets), (C.c_int8 * max(1, len(signs)))(*signs)
+
+
+# ---- backends -----------------------------------------------------------------------------
+
+class Backend:
+    """A backend given by its function table (`mm_backend`); `table` is a BackendTable."""
+
+    table = None
+
+    def coefficient_bound(self):  # Limits::coefficient_bound (transform.rs:13)
+        return self.table.bound_lo | (self.table.bound_hi << 64)
+
+    def supported_lengths(self):  # Limits::supported_lengths (transform.rs:15)
+        out, length = [], self.table.min_len
+        while length and length <= self.table.max_len:
+            out.append(length)
+            length *= 2
+        return out
+
+    def full_k(self):  # Limits::full_k (transform.rs:17)
+        return bool(self.table.full_k)
+
+
+class Hip(Backend):
+    """The MI355X backend (replaces `Gpu`/`GpuOptions`, gpu/mod.rs:28-111)."""
+
+    def __init__(self, device=0, force_global=False, wraparound=True, max_launch_ms=0):
+        self.options = Options(device, int(bool(force_global)), int(bool(wraparound)), max_launch_ms)
+        self.table = BackendTable()
+        if lib.mm_hip_open(C.byref(self.options), C.byref(self.table)) != 0:
+            raise MontmulError((lib.mm_last_error() or b"").decode())
+        self._open = True
+
+    @staticmethod
+    def device_count():
+        n = lib.mm_device_count()
+        if n < 0:
+            raise MontmulError((lib.mm_last_error() or b"").decode())
+        return n
+
+    def close(self):
+        if getattr(self, "_open", False):
+            lib.mm_hip_close(C.byref(self.table))
+            self._open = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def free_memory(self):
+        return lib.mm_free_memory(self.table.self)
+
+    def cu_count(self):
+        return lib.mm_cu_count(self.table.self)
+
+    def max_lds(self):
+        return lib.mm_max_lds(self.table.self)
+
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        lib.mm_device_name(self.table.self, buf, 256)
+        return buf.value.decode()
+
+    # GPU-only extras on a batch's state.
+    @staticmethod
+    def plan_of(batch):
+        p = PlanInfo()
+        lib.mm_plan(batch.state, C.byref(p))
+        return {"global": bool(p.global_), "threads": p.threads, "lds_bytes": p.lds_bytes,
+                "slots_per_wg": p.slots_per_wg}
+
+    @staticmethod
+    def run_bare(batch, bits):
+        r = lib.mm_run_bare(batch.state, bits)
+        _check_run(r, lambda: (lib.mm_last_error() or b"").decode())
+
+    @staticmethod
+    def push(batch):
+        if lib.mm_push(batch.state) != 0:
+            raise MontmulError((lib.mm_last_error() or b"").decode())
+
+    @staticmethod
+    def last_run_kernel_ms(batch):
+        ms, n = C.c_double(), C.c_uint32()
+        lib.mm_last_run_kernel_ms(batch.state, C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+
+def _check_run(r, msg):
+    if r == MM_OK:
+        return
+    if r >= 0:
+        raise CarryError(msg(), r)
+    raise MontmulError(msg())
+
+
+# ---- host API -----------------------------------------------------------------------------
+
+class Params:
+    """`Params` (batch.rs:9-31)."""
+
+    W = 32
+
+    def __init__(self, k, length):
+        self.k, self.len = k, length
+
+    def __repr__(self):
+        return f"Params(k={self.k}, len={self.len})"
+
+    def __eq__(self, o):
+        return isinstance(o, Params) and (self.k, self.len) == (o.k, o.len)
+
+    @staticmethod
+    def for_bits(bits, lengths):
+        lengths = list(lengths)
+        k, ln = C.c_uint32(), C.c_uint32()
+        if lib.mm_params_for_bits(bits, min(lengths), max(lengths), C.byref(k), C.byref(ln)) != 0:
+            raise MontmulError(_host_error())
+        return Params(k.value, ln.value)
+
+    def check_exact(self, c, bound):
+        if lib.mm_params_check_exact(self.k, c, bound & (2**64 - 1), bound >> 64) != 0:
+            raise MontmulError(_host_error())
+
+
+class ModBatch:
+    """`ModBatch<B>` (batch.rs:53-185) over any backend table."""
+
+    def __init__(self, backend, moduli, base):
+        self.backend = backend
+        moduli = [int(m) for m in moduli]
+        limbs, offsets, signs = _pack_ints(moduli)
+        h = C.c_void_p()
+        if lib.mm_modbatch_new(C.byref(backend.table), len(moduli), limbs, offsets, signs, base, C.byref(h)) != 0:
+            raise MontmulError(_host_error())
+        self._h = h
+        self._moduli = moduli
+        k, ln = C.c_uint32(), C.c_uint32()
+        lib.mm_modbatch_params(h, C.byref(k), C.byref(ln))
+        self._params = Params(k.value, ln.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mm_modbatch_free(h)
+            self._h = None
+
+    def free(self):
+        self.__del__()
+
+    @property
+    def state(self):
+        return lib.mm_modbatch_state(self._h)
+
+    def params(self):
+        return self._params
+
+    def slots(self):
+        return len(self._moduli)
+
+    def modulus(self, i):
+        return self._moduli[i]
+
+    def load(self, i, x):
+        x = int(x)
+        buf = to_limbs(x)
+        if lib.mm_modbatch_load(self._h, i, _sign(x), buf, len(buf)) != 0:
+            raise MontmulError(_host_error())
+
+    def load_raw(self, i, xm):
+        xm = int(xm)
+        buf = to_limbs(xm)
+        if lib.mm_modbatch_load_raw(self._h, i, _sign(xm), buf, len(buf)) != 0:
+            raise MontmulError(_host_error())
+
+    def set_exponent(self, i, e):
+        e = int(e)
+        buf = to_limbs(e)
+        if lib.mm_modbatch_set_exponent(self._h, i, _sign(e), buf, len(buf)) != 0:
+            raise MontmulError(_host_error())
+
+    def run(self, bits):
+        _check_run(lib.mm_modbatch_run(self._h, bits), _host_error)
+
+    def remaining(self, i):
+        return lib.mm_modbatch_remaining(self._h, i)
+
+    def mul(self, out, a, b):
+        _check_run(lib.mm_modbatch_mul(self._h, out, a, b), _host_error)
+
+    def read_raw(self, i):
+        buf = (C.c_uint32 * self._params.k)()
+        if lib.mm_modbatch_read_raw(self._h, i, buf) != 0:
+            raise MontmulError(_host_error())
+        return from_limbs(buf)
+
+    def read(self, i):
+        buf = (C.c_uint32 * self._params.k)()
+        if lib.mm_modbatch_read(self._h, i, buf) != 0:
+            raise MontmulError(_host_error())
+        return from_limbs(buf)
+
+    def is_one(self, i):
+        r = lib.mm_modbatch_is_one(self._h, i)
+        if r < 0:
+            raise MontmulError(_host_error())
+        return bool(r)
+
+
+def fermat(backend, moduli, base, chunk):
+    """Base-`base` Fermat PRP test of every modulus (batch.rs:187-198)."""
+    moduli = [int(m) for m in moduli]
+    limbs, offsets, signs = _pack_ints(moduli)
+    out = (C.c_uint8 * max(1, len(moduli)))()
+    if lib.mm_fermat(C.byref(backend.table), len(moduli), limbs, offsets, signs, base, chunk, out) != 0:
+        raise MontmulError(_host_error())
+    return [bool(out[i]) for i in range(len(moduli))]
+
+
+def score_workload(bits, slots, ebits, seed):
+    """The score workload (score.rs:45-61): moduli and exponents from GMP's seeded generator."""
+    nl, el = (bits + 31) // 32, (ebits + 31) // 32
+    mod = (C.c_uint32 * (slots * nl))()
+    exp = (C.c_uint32 * (slots * el))()
+    if lib.mm_score_workload(bits, slots, ebits, seed, mod, exp) != 0:
+        raise MontmulError(_host_error())
+    mb, eb = bytes(mod), bytes(exp)
+    moduli = [int.from_bytes(mb[4 * nl * i:4 * nl * (i + 1)], "little") for i in range(slots)]
+    exps = [int.from_bytes(eb[4 * el * i:4 * el * (i + 1)], "little") for i in range(slots)]
+    return moduli, exps

@@ -1,0 +1,796 @@
+// mm_hip.hip - the HIP backend: kernels, per-length tables, batch state and the C ABI
+// declared in include/montmul_hip.h.
+//
+// Replaces the reference's `Gpu` backend (montmul/src/gpu/mod.rs, gpu/device.rs, gpu/plan.rs)
+// and its kernels (montmul/kernels/src/lib.rs). One workgroup owns one slot and runs whole
+// steps on the device; the host uploads loaded values and exponents, launches, and checks the
+// per-slot error words.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/montmul_hip.h"
+#include "mm_step.h"
+
+namespace mm {
+
+struct KArgs {
+    Tab t;
+    HalfTab ht;
+    Slots s;
+};
+
+// ---- Kernels ---------------------------------------------------------------------------------
+
+// Up to `bits` steps on slot first + blockIdx.x (reference: `steps`, kernels/src/lib.rs:883-927).
+// BARE runs the squaring alone and leaves x unchanged (lib.rs:949-961).
+template <bool GLOBAL, bool WRAP, bool BARE>
+__global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_t bits) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const Slots& S = A.s;
+    const uint32_t slot = first + blockIdx.x;
+    const uint32_t n = A.t.n, k = S.k;
+    Work w;
+    if (GLOBAL) {
+        w.a1 = S.work1 + (size_t)slot * n;
+        w.a2 = S.work2 + (size_t)slot * n;
+        w.lt = S.lim + (size_t)slot * 3 * k;
+        w.ls = w.lt + 2 * k;
+        w.sc = (uint64_t*)lds;
+    } else {
+        w.a2 = (F2*)lds;
+        w.a1 = (F1*)(lds + 16 * (size_t)n);
+        w.lt = (uint32_t*)(lds + 24 * (size_t)n);
+        w.ls = w.lt + 2 * k;
+        w.sc = (uint64_t*)(lds + ((24 * (size_t)n + 12 * (size_t)k + 7) & ~(size_t)7));
+    }
+    uint32_t* x = S.x + (size_t)slot * k;
+    const uint32_t* ex = S.exp + (size_t)slot * S.ew;
+    const uint32_t nb = S.nbits[slot];
+    uint32_t pos = S.pos[slot];
+    uint32_t bad = 0;
+    for (uint32_t it = 0; it < bits && pos < nb; it++, pos++) {
+        const uint32_t bit = (ex[pos >> 5] >> (31 - (pos & 31))) & 1;
+        const uint32_t c = bit ? S.base : 1;
+        bad |= square(w, x, k, c, A.t) != 0;
+        if (!BARE)
+            bad |= redc<WRAP>(w, x, k, A.t, A.ht, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
+                              S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
+    }
+    if (tid() == 0) {
+        S.pos[slot] = pos;
+        if (bad) S.err[slot] = 1;
+    }
+}
+
+// Weighted forward transform of k-limb strings into out (n elements per vector), in place in
+// the output (reference: `t_load_forward`, lib.rs:1040-1049; used for N' and, without the
+// wraparound correction, N).
+__global__ __launch_bounds__(1024) void k_load_forward(Tab t, const uint32_t* limbs, uint32_t k, F1* out1, F2* out2) {
+    const uint32_t v = blockIdx.x, n = t.n;
+    F1* a1 = out1 + (size_t)v * n;
+    F2* a2 = out2 + (size_t)v * n;
+    load_weighted(a1, a2, limbs + (size_t)v * k, k, t);
+    __syncthreads();
+    dif(a1, t, t.dtw1);
+    dif(a2, t, t.dtw2);
+}
+
+// Per-slot factors A | B of the half path for N (k = 2h limbs): N-hat as in the reference's
+// `half_nhat` (cpu.rs:239-249) and the factors of `half_factors` (cpu.rs:261-277), computed on
+// the device. tmp holds 2h elements per slot; ab receives A at [0, h) and B at [h, 2h).
+template <class F>
+__device__ void half_factors_dev(F* z, F* s, F* ab, const F* u, const F* iu, uint32_t h, F c) {
+    // s_p = half_nhat at p: (x, y) = (e + wo, e - wo), e = z_p + conj z_q, wo = (z_p - conj z_q)(-i) u_p.
+    for (uint32_t p = tid(); p < h; p += nthreads()) {
+        F zp = z[p], zq = cconj(z[partner(p)]);
+        F e = cadd(zp, zq), wo = cmul(cmul_mi(csub(zp, zq)), u[p]);
+        F xx = cadd(e, wo), yy = csub(e, wo);
+        s[p] = p == 0 ? cadd(cmul(xx, c), cmul_i(cmul(yy, c))) : cmul(xx, c);
+    }
+    __syncthreads();
+    for (uint32_t p = tid(); p < h; p += nthreads()) {
+        F nx, ny;
+        if (p == 0) {
+            nx = s[0];
+            nx.im = 0;
+            ny = s[0];
+            ny.re = s[0].im;
+            ny.im = 0;
+        } else {
+            nx = s[p];
+            ny = cconj(s[partner(p)]);
+        }
+        F one{};
+        one.re = 1;
+        F i_iu = cmul_i(iu[p]);
+        F X = cmul(nx, cadd(one, i_iu)), Y = cmul(ny, csub(one, i_iu));
+        ab[p] = cadd(X, Y);
+        ab[h + p] = cmul(cmul_mi(csub(X, Y)), u[p]);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_half_factors(HalfTab ht, const uint32_t* nlimbs, uint32_t k, F1* tmp1, F2* tmp2,
+                                                       F1* ab1, F2* ab2, F1 c1, F2 c2) {
+    const uint32_t v = blockIdx.x, h = ht.t.n;
+    F1* z1 = tmp1 + (size_t)v * 2 * h;
+    F2* z2 = tmp2 + (size_t)v * 2 * h;
+    const uint32_t* nl = nlimbs + (size_t)v * k;
+    for (uint32_t j = tid(); j < h; j += nthreads()) {
+        uint32_t re = nl[2 * j], im = nl[2 * j + 1];
+        z1[j] = F1{mod1(re), mod1(im)};
+        z2[j] = F2{(uint64_t)re, (uint64_t)im};
+    }
+    __syncthreads();
+    dif(z1, ht.t, ht.t.dtw1);
+    dif(z2, ht.t, ht.t.dtw2);
+    half_factors_dev(z1, z1 + h, ab1 + (size_t)v * 2 * h, ht.u1, ht.iu1, h, c1);
+    half_factors_dev(z2, z2 + h, ab2 + (size_t)v * 2 * h, ht.u2, ht.iu2, h, c2);
+}
+
+// x[out] <- x[a] x[b] R^-1 with out's modulus (reference: `mul_slots`, lib.rs:963-988). One
+// workgroup; vectors in device memory (mb: F(x_b), w: the working vector), limbs in lim.
+template <bool WRAP>
+__global__ __launch_bounds__(1024) void k_mul(KArgs A, uint32_t out, uint32_t ia, uint32_t ib, F1* mb1, F2* mb2, F1* w1,
+                                              F2* w2, uint32_t* lim) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const Slots& S = A.s;
+    const uint32_t n = A.t.n, k = S.k;
+    Work w{w1, w2, lim, lim + 2 * k, (uint64_t*)lds};
+    load_weighted(mb1, mb2, S.x + (size_t)ib * k, k, A.t);
+    __syncthreads();
+    dif(mb1, A.t, A.t.dtw1);
+    dif(mb2, A.t, A.t.dtw2);
+    load_weighted(w.a1, w.a2, S.x + (size_t)ia * k, k, A.t);
+    __syncthreads();
+    dif(w.a1, A.t, A.t.dtw1);
+    dif(w.a2, A.t, A.t.dtw2);
+    pointwise_const(w, mb1, mb2, n);
+    dit(w.a1, A.t, A.t.itw1);
+    dit(w.a2, A.t, A.t.itw2);
+    unweight(w.a1, w.a2, A.t);
+    __syncthreads();
+    uint32_t bad = block_carry<CARRY_E>(ProductCoef{w.a1, w.a2, n, A.t.crt_inv}, 2 * k, StoreLimbs{w.lt}, w.sc, 0) != 0;
+    bad |= redc<WRAP>(w, S.x + (size_t)out * k, k, A.t, A.ht, S.nphat1 + (size_t)out * n, S.nphat2 + (size_t)out * n,
+                      S.nhat1 + (size_t)out * n, S.nhat2 + (size_t)out * n);
+    if (tid() == 0 && bad) S.err[out] = 1;
+}
+
+// ---- Host-side field helpers (table generation) ----------------------------------------------
+template <class F> F cone() { F r{}; r.re = 1; return r; }
+template <class F> F ci() { F r{}; r.im = 1; return r; }
+template <class F> F cpow(F b, u128 e) {
+    F r = cone<F>();
+    while (e) {
+        if (e & 1) r = cmul(r, b);
+        b = cmul(b, b);
+        e >>= 1;
+    }
+    return r;
+}
+inline F1 cinv(F1 a) {
+    uint32_t nrm = add1(mul1(a.re, a.re), mul1(a.im, a.im));
+    uint32_t r = 1, b = nrm;
+    for (uint64_t e = P1 - 2; e; e >>= 1) { if (e & 1) r = mul1(r, b); b = mul1(b, b); }
+    return F1{mul1(a.re, r), mul1(neg1(a.im), r)};
+}
+inline F2 cinv(F2 a) {
+    uint64_t nrm = add2(mul2(a.re, a.re), mul2(a.im, a.im));
+    uint64_t r = 1, b = nrm;
+    for (uint64_t e = P2 - 2; e; e >>= 1) { if (e & 1) r = mul2(r, b); b = mul2(b, b); }
+    return F2{mul2(a.re, r), mul2(neg2(a.im), r)};
+}
+template <class F> struct Ord;
+template <> struct Ord<F1> { static constexpr uint32_t E = 32; static constexpr uint64_t COF = (1ull << 30) - 1; };
+template <> struct Ord<F2> { static constexpr uint32_t E = 62; static constexpr uint64_t COF = (1ull << 60) - 1; };
+// The generator of the 2-Sylow subgroup that the reference uses (field.rs:111-120): the first
+// (a + i)^cofactor, a = 1, 2, ..., whose order is the full 2^E.
+template <class F> F sylow2() {
+    for (uint32_t a = 1;; a++) {
+        F z{};
+        z.re = a;
+        z.im = 1;
+        F g = cpow(z, (u128)Ord<F>::COF);
+        if (!ceq(cpow(g, (u128)1 << (Ord<F>::E - 1)), cone<F>())) return g;
+    }
+}
+template <class F> const F& gen() { static const F g = sylow2<F>(); return g; }
+
+inline uint32_t ilog2(uint64_t n) { uint32_t l = 0; while ((1ull << l) < n) l++; return l; }
+
+// Host copies of one field's tables at length n.
+template <class F> struct HostTab {
+    std::vector<F> w, iw, dtw, itw, u, iu;
+    bool plus = true;
+};
+
+// Roots as the reference picks them (field.rs:237-267): omega = g^(2^(E - log n)) and
+// zeta = g^(2^(E - log n - 2)), inverted unless zeta^n = i.
+template <class F> HostTab<F> make_tab(uint32_t n, bool weights, bool unpack) {
+    HostTab<F> T;
+    const uint32_t logn = ilog2(n), E = Ord<F>::E;
+    const F g = gen<F>();
+    const F omega = cpow(g, (u128)1 << (E - logn));
+    if (n >= 4) {
+        F i4 = cpow(omega, (u128)(n / 4));
+        F mi{};
+        mi.im = std::is_same<F, F1>::value ? (uint64_t)P1 - 1 : P2 - 1;
+        if (ceq(i4, ci<F>())) T.plus = true;
+        else if (ceq(i4, mi)) T.plus = false;
+        else { fprintf(stderr, "montmul_hip: omega^(n/4) is not +-i\n"); abort(); }
+    }
+    if (weights) {
+        F zeta = cpow(g, (u128)1 << (E - logn - 2));
+        if (!ceq(cpow(zeta, (u128)n), ci<F>())) zeta = cinv(zeta);
+        F nn{};
+        nn.re = n;
+        F ninv = cinv(nn);
+        F zi = cinv(zeta);
+        T.w.resize(n);
+        T.iw.resize(n);
+        F a = cone<F>(), b = ninv;
+        for (uint32_t j = 0; j < n; j++) {
+            T.w[j] = a;
+            T.iw[j] = b;
+            a = cmul(a, zeta);
+            b = cmul(b, zi);
+        }
+    }
+    const uint32_t npass = logn / 2;
+    for (uint32_t p = 0; p < npass; p++) {
+        uint32_t q = dif_pass_q(logn, p);
+        F W = cpow(omega, (u128)1 << (2 * p)), x = cone<F>();
+        for (uint32_t j = 0; j < q; j++) {
+            F x2 = cmul(x, x);
+            T.dtw.push_back(x);
+            T.dtw.push_back(x2);
+            T.dtw.push_back(cmul(x2, x));
+            x = cmul(x, W);
+        }
+    }
+    const F oinv = cinv(omega);
+    for (uint32_t p = 0; p < npass; p++) {
+        uint32_t q = dit_pass_q(logn, p);
+        F V = cpow(oinv, (u128)(n / (4 * q))), x = cone<F>();
+        for (uint32_t j = 0; j < q; j++) {
+            F x2 = cmul(x, x);
+            T.itw.push_back(x);
+            T.itw.push_back(x2);
+            T.itw.push_back(cmul(x2, x));
+            x = cmul(x, V);
+        }
+    }
+    if (unpack) {
+        // u[p] = w^rev(p), iu[p] = w^-rev(p), w a primitive 2n-th root (reference: Unpack,
+        // cpu.rs:165-179, for the half length n = h).
+        const F w = cpow(g, (u128)1 << (E - (logn + 1))), wi = cinv(w);
+        std::vector<F> pw(n), pwi(n);
+        F a = cone<F>(), b = cone<F>();
+        for (uint32_t j = 0; j < n; j++) { pw[j] = a; pwi[j] = b; a = cmul(a, w); b = cmul(b, wi); }
+        T.u.resize(n);
+        T.iu.resize(n);
+        for (uint32_t p = 0; p < n; p++) {
+            uint32_t r = 0;
+            for (uint32_t bit = 0; bit < logn; bit++) if (p >> bit & 1) r |= 1u << (logn - 1 - bit);
+            T.u[p] = pw[r];
+            T.iu[p] = pwi[r];
+        }
+    }
+    return T;
+}
+
+}  // namespace mm
+
+using namespace mm;
+
+// ---- Backend state ----------------------------------------------------------------------------
+namespace {
+
+thread_local std::string g_err;
+int fail(const std::string& m) { g_err = m; return 1; }
+int hip_fail(const char* what, hipError_t e) { return fail(std::string(what) + ": " + hipGetErrorString(e)); }
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hip_fail(#x, e_); } while (0)
+
+struct DevTables {
+    uint32_t n = 0;
+    Tab t{};
+    HalfTab ht{};  // tables at length n (the half path of length 2n uses these)
+    std::vector<void*> bufs;
+    int device = 0;
+    ~DevTables() {
+        hipSetDevice(device);
+        for (void* b : bufs) hipFree(b);
+    }
+};
+
+struct Ctx {
+    mm_options opts{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    size_t max_lds = 0;
+    std::string name;
+    std::mutex mu;
+    std::map<uint32_t, std::shared_ptr<DevTables>> tabs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+template <class T> int upload_vec(DevTables& D, const std::vector<T>& v, const T** out) {
+    *out = nullptr;
+    if (v.empty()) return 0;
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, v.size() * sizeof(T)));
+    D.bufs.push_back(p);
+    HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T*)p;
+    return 0;
+}
+
+// Tables at complex length n, built and uploaded on first use (reference: Gpu::tables,
+// gpu/mod.rs:186-190). The same object serves as the half tables of length-2n transforms.
+int get_tables(Ctx* C, uint32_t n, std::shared_ptr<DevTables>* out) {
+    std::lock_guard<std::mutex> g(C->mu);
+    auto it = C->tabs.find(n);
+    if (it != C->tabs.end()) { *out = it->second; return 0; }
+    auto D = std::make_shared<DevTables>();
+    D->device = C->device;
+    D->n = n;
+    HostTab<F1> h1 = make_tab<F1>(n, true, true);
+    HostTab<F2> h2 = make_tab<F2>(n, true, true);
+    if (h1.plus != h2.plus && n >= 4) return fail("fourth roots differ between fields");
+    Tab& t = D->t;
+    t.n = n;
+    t.logn = ilog2(n);
+    t.i_is_plus = h1.plus;
+    t.crt_inv = 0;
+    {
+        // (P2 mod P1)^-1 mod P1 (field.rs:283-286).
+        uint32_t b = (uint32_t)(P2 % P1), r = 1;
+        for (uint64_t e = P1 - 2; e; e >>= 1) { if (e & 1) r = mul1(r, b); b = mul1(b, b); }
+        t.crt_inv = r;
+    }
+    if (upload_vec(*D, h1.w, &t.w1) || upload_vec(*D, h1.iw, &t.iw1) || upload_vec(*D, h2.w, &t.w2) ||
+        upload_vec(*D, h2.iw, &t.iw2) || upload_vec(*D, h1.dtw, &t.dtw1) || upload_vec(*D, h2.dtw, &t.dtw2) ||
+        upload_vec(*D, h1.itw, &t.itw1) || upload_vec(*D, h2.itw, &t.itw2))
+        return 1;
+    D->ht.t = t;
+    if (upload_vec(*D, h1.u, &D->ht.u1) || upload_vec(*D, h1.iu, &D->ht.iu1) || upload_vec(*D, h2.u, &D->ht.u2) ||
+        upload_vec(*D, h2.iu, &D->ht.iu2))
+        return 1;
+    C->tabs[n] = D;
+    *out = D;
+    return 0;
+}
+
+struct State {
+    Ctx* C = nullptr;
+    uint32_t k = 0, len = 0, n = 0, base = 0, slots = 0;
+    bool global = false, wrap = true;
+    uint32_t threads = 0;
+    size_t lds = 0;
+    std::shared_ptr<DevTables> tab, half;
+    // device buffers
+    uint32_t *d_x = nullptr, *d_exp = nullptr, *d_nbits = nullptr, *d_pos = nullptr, *d_err = nullptr, *d_lim = nullptr;
+    F1 *d_nphat1 = nullptr, *d_nhat1 = nullptr, *d_work1 = nullptr;
+    F2 *d_nphat2 = nullptr, *d_nhat2 = nullptr, *d_work2 = nullptr;
+    uint32_t ew = 0;
+    // host mirrors
+    std::vector<uint32_t> h_x;
+    bool x_host_valid = true, x_dirty = false;
+    std::vector<std::vector<uint32_t>> exps;
+    std::vector<uint32_t> nbits, pos;
+    bool exp_dirty = false;
+    double ms_per_slot_step[2] = {0, 0};
+    double last_ms = 0;
+    uint32_t last_launches = 0;
+    ~State() {
+        hipSetDevice(C->device);
+        for (void* p : {(void*)d_x, (void*)d_exp, (void*)d_nbits, (void*)d_pos, (void*)d_err, (void*)d_lim,
+                        (void*)d_nphat1, (void*)d_nhat1, (void*)d_work1, (void*)d_nphat2, (void*)d_nhat2, (void*)d_work2})
+            if (p) hipFree(p);
+    }
+    KArgs args() const {
+        KArgs A{};
+        A.t = tab->t;
+        if (half) A.ht = half->ht;
+        Slots& s = A.s;
+        s.x = d_x;
+        s.nphat1 = d_nphat1; s.nphat2 = d_nphat2;
+        s.nhat1 = d_nhat1; s.nhat2 = d_nhat2;
+        s.work1 = d_work1; s.work2 = d_work2;
+        s.lim = d_lim;
+        s.exp = d_exp; s.nbits = d_nbits; s.pos = d_pos; s.err = d_err;
+        s.k = k; s.ew = ew ? ew : 1; s.base = base; s.slots = slots;
+        return A;
+    }
+};
+
+// Threads per workgroup: one radix-4 group per thread, 64 to 1024.
+uint32_t plan_threads(uint32_t n) { return std::min<uint32_t>(1024, std::max<uint32_t>(64, n / 4)); }
+
+template <class T> int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return 0;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return hip_fail("hipMalloc", e);
+    return 0;
+}
+
+typedef void (*StepKernel)(KArgs, uint32_t, uint32_t);
+StepKernel pick_step(bool global, bool wrap, bool bare) {
+    if (global) {
+        if (bare) return wrap ? k_steps<true, true, true> : k_steps<true, false, true>;
+        return wrap ? k_steps<true, true, false> : k_steps<true, false, false>;
+    }
+    if (bare) return wrap ? k_steps<false, true, true> : k_steps<false, false, true>;
+    return wrap ? k_steps<false, true, false> : k_steps<false, false, false>;
+}
+
+int set_lds_limit(Ctx* C) {
+    const StepKernel ks[] = {k_steps<true, true, true>,   k_steps<true, false, true>,  k_steps<true, true, false>,
+                             k_steps<true, false, false>, k_steps<false, true, true>,  k_steps<false, false, true>,
+                             k_steps<false, true, false>, k_steps<false, false, false>};
+    for (StepKernel k : ks)
+        HIPCHK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C->max_lds));
+    return 0;
+}
+
+int push(State* S) {
+    if (S->x_dirty) {
+        HIPCHK(hipMemcpyAsync(S->d_x, S->h_x.data(), S->h_x.size() * 4, hipMemcpyHostToDevice, S->C->stream));
+        S->x_dirty = false;
+    }
+    if (S->exp_dirty) {
+        uint32_t ew = 1;
+        for (auto& e : S->exps) ew = std::max<uint32_t>(ew, (uint32_t)e.size());
+        if (ew > S->ew || !S->d_exp) {
+            HIPCHK(hipStreamSynchronize(S->C->stream));
+            if (S->d_exp) hipFree(S->d_exp);
+            S->d_exp = nullptr;
+            if (dalloc(&S->d_exp, (size_t)S->slots * ew)) return 1;
+            S->ew = ew;
+        }
+        std::vector<uint32_t> words((size_t)S->slots * S->ew, 0);
+        for (uint32_t i = 0; i < S->slots; i++)
+            std::copy(S->exps[i].begin(), S->exps[i].end(), words.begin() + (size_t)i * S->ew);
+        HIPCHK(hipMemcpyAsync(S->d_exp, words.data(), words.size() * 4, hipMemcpyHostToDevice, S->C->stream));
+        HIPCHK(hipMemcpyAsync(S->d_nbits, S->nbits.data(), S->slots * 4, hipMemcpyHostToDevice, S->C->stream));
+        HIPCHK(hipMemcpyAsync(S->d_pos, S->pos.data(), S->slots * 4, hipMemcpyHostToDevice, S->C->stream));
+        HIPCHK(hipStreamSynchronize(S->C->stream));
+        S->exp_dirty = false;
+    }
+    return 0;
+}
+
+int pull_x(State* S) {
+    if (S->x_host_valid) return 0;
+    HIPCHK(hipMemcpyAsync(S->h_x.data(), S->d_x, S->h_x.size() * 4, hipMemcpyDeviceToHost, S->C->stream));
+    HIPCHK(hipStreamSynchronize(S->C->stream));
+    S->x_host_valid = true;
+    return 0;
+}
+
+// First slot whose error word is set (clearing all), -1 if none, -2 on a device error.
+int64_t check_err(State* S, const char* what) {
+    std::vector<uint32_t> err(S->slots);
+    hipError_t e = hipMemcpyAsync(err.data(), S->d_err, S->slots * 4, hipMemcpyDeviceToHost, S->C->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->C->stream);
+    if (e != hipSuccess) { hip_fail(what, e); return MM_EFAULT; }
+    for (uint32_t i = 0; i < S->slots; i++)
+        if (err[i]) {
+            if (hipMemsetAsync(S->d_err, 0, S->slots * 4, S->C->stream) != hipSuccess ||
+                hipStreamSynchronize(S->C->stream) != hipSuccess) {
+                fail(std::string(what) + ": clearing error words failed");
+                return MM_EFAULT;
+            }
+            g_err = std::string(what) + ": carry out of a product that must fit, slot " + std::to_string(i);
+            return (int64_t)i;
+        }
+    return MM_OK;
+}
+
+// Runs up to `bits` steps on every slot with bits left (reference: Gpu::steps, gpu/mod.rs:113-158).
+int64_t steps(State* S, uint64_t bits, bool bare) {
+    Ctx* C = S->C;
+    if (hipSetDevice(C->device) != hipSuccess) { fail("hipSetDevice failed"); return MM_EFAULT; }
+    uint64_t most = 0;
+    for (uint32_t i = 0; i < S->slots; i++) most = std::max<uint64_t>(most, S->nbits[i] - S->pos[i]);
+    uint64_t left = std::min(bits, most);
+    S->last_ms = 0;
+    S->last_launches = 0;
+    if (left == 0) return MM_OK;
+    if (push(S)) return MM_EFAULT;
+    KArgs A = S->args();
+    StepKernel kern = pick_step(S->global, S->wrap, bare);
+    while (left > 0) {
+        // Steps per launch: all of them, or about max_launch_ms worth from the latest rate.
+        uint64_t s = left;
+        if (C->opts.max_launch_ms > 0) {
+            double e = S->ms_per_slot_step[bare];
+            s = e > 0 ? (uint64_t)std::max(1.0, C->opts.max_launch_ms / (e * std::max<uint32_t>(1, S->slots))) : 1;
+            s = std::min(s, left);
+        }
+        s = std::min<uint64_t>(s, 0xffffffffu);
+        hipEventRecord(C->ev0, C->stream);
+        hipLaunchKernelGGL(kern, dim3(S->slots), dim3(S->threads), S->lds, C->stream, A, 0u, (uint32_t)s);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) { hip_fail("step kernel launch", le); return MM_EFAULT; }
+        hipEventRecord(C->ev1, C->stream);
+        hipError_t se = hipEventSynchronize(C->ev1);
+        if (se != hipSuccess) { hip_fail("step kernel", se); return MM_EFAULT; }
+        float ms = 0;
+        hipEventElapsedTime(&ms, C->ev0, C->ev1);
+        S->last_ms += ms;
+        S->last_launches++;
+        S->ms_per_slot_step[bare] = std::max(1e-6, (double)ms) / ((double)s * std::max<uint32_t>(1, S->slots));
+        int64_t r = check_err(S, bare ? "run_bare" : "run");
+        for (uint32_t i = 0; i < S->slots; i++) S->pos[i] += (uint32_t)std::min<uint64_t>(s, S->nbits[i] - S->pos[i]);
+        if (!bare) S->x_host_valid = false;
+        if (r != MM_OK) return r;
+        left -= s;
+    }
+    return MM_OK;
+}
+
+}  // namespace
+
+// ---- C ABI -------------------------------------------------------------------------------------
+extern "C" {
+
+const char* mm_last_error(void) { return g_err.c_str(); }
+
+int mm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { fail("hipGetDeviceCount failed"); return -1; }
+    return n;
+}
+
+int mm_hip_open(const mm_options* opts, mm_backend* out) {
+    memset(out, 0, sizeof(*out));
+    auto C = new Ctx();
+    C->opts = opts ? *opts : mm_options{0, 0, 1, 0};
+    C->device = C->opts.device;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || C->device < 0 || C->device >= count) {
+        delete C;
+        return fail("no HIP device " + std::to_string(opts ? opts->device : 0));
+    }
+    hipDeviceProp_t p;
+    if (hipSetDevice(C->device) != hipSuccess || hipGetDeviceProperties(&p, C->device) != hipSuccess) {
+        delete C;
+        return fail("cannot query HIP device");
+    }
+    C->cus = p.multiProcessorCount;
+    C->max_lds = p.sharedMemPerBlock;
+    C->name = std::string(p.gcnArchName);
+    if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&C->ev0) != hipSuccess ||
+        hipEventCreate(&C->ev1) != hipSuccess || set_lds_limit(C)) {
+        delete C;
+        return 1;
+    }
+    out->self = C;
+    out->create = mm_create;
+    out->destroy = mm_destroy;
+    out->load = mm_load;
+    out->set_exponent = mm_set_exponent;
+    out->run = mm_run;
+    out->mul = mm_mul;
+    out->read = mm_read;
+    out->remaining = mm_remaining;
+    out->last_error = mm_last_error;
+    // Limits (gpu/mod.rs:272-283): bound Q = P1 * P2, lengths 2^3..2^24, full_k = wraparound.
+    u128 Q = (u128)P1 * P2;
+    out->bound_lo = (uint64_t)Q;
+    out->bound_hi = (uint64_t)(Q >> 64);
+    out->min_len = 8;
+    out->max_len = 1u << 24;
+    out->full_k = C->opts.wraparound ? 1 : 0;
+    return 0;
+}
+
+void mm_hip_close(mm_backend* be) {
+    Ctx* C = (Ctx*)be->self;
+    if (!C) return;
+    hipSetDevice(C->device);
+    C->tabs.clear();
+    if (C->ev0) hipEventDestroy(C->ev0);
+    if (C->ev1) hipEventDestroy(C->ev1);
+    if (C->stream) hipStreamDestroy(C->stream);
+    delete C;
+    be->self = nullptr;
+}
+
+// Backend::create (reference: Gpu::create, gpu/mod.rs:325-370).
+int mm_create(void* self, const mm_batch_setup* st, void** out) {
+    Ctx* C = (Ctx*)self;
+    *out = nullptr;
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    if (st->len < 8 || (st->len & (st->len - 1)) || st->len > (1u << 24)) return fail("unsupported transform length");
+    if (st->slots == 0) return fail("empty batch");
+    const uint32_t n = st->len / 2, k = st->k;
+    if (k > n) return fail("k exceeds L/2");
+    const bool wrap = C->opts.wraparound != 0;
+    if (wrap && k != n) return fail("wraparound needs k = L/2");
+    std::unique_ptr<State> S(new State());
+    S->C = C;
+    S->k = k; S->len = st->len; S->n = n; S->base = st->base; S->slots = st->slots; S->wrap = wrap;
+    S->threads = plan_threads(n);
+    const uint64_t resident = lds_bytes_resident(n, k, S->threads);
+    S->global = C->opts.force_global || resident > C->max_lds;
+    S->lds = S->global ? lds_scratch_bytes(S->threads) : resident;
+    // Device memory: x, N'-hat, N-hat or A|B, work vectors and limbs in global mode, exponents
+    // of up to k words, per-slot words, tables (about 6n elements per field, twice), a margin.
+    const uint64_t per_slot = 4ull * k + 48ull * n + (S->global ? 24ull * n + 12ull * k : 0) + 16 + 4ull * k;
+    const uint64_t need = (uint64_t)st->slots * per_slot + 300ull * n + 48ull * n * 2 + (64ull << 20);
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return fail("hipMemGetInfo failed");
+    if (need > freeb) return fail("batch needs " + std::to_string(need) + " bytes, " + std::to_string(freeb) + " free");
+    if (get_tables(C, n, &S->tab)) return 1;
+    if (wrap && get_tables(C, n / 2, &S->half)) return 1;
+    const size_t sl = st->slots;
+    if (dalloc(&S->d_x, sl * k) || dalloc(&S->d_nbits, sl) || dalloc(&S->d_pos, sl) || dalloc(&S->d_err, sl) ||
+        dalloc(&S->d_nphat1, sl * n) || dalloc(&S->d_nphat2, sl * n) || dalloc(&S->d_nhat1, sl * n) ||
+        dalloc(&S->d_nhat2, sl * n) || dalloc(&S->d_exp, sl))
+        return 1;
+    S->ew = 1;
+    if (S->global && (dalloc(&S->d_work1, sl * n) || dalloc(&S->d_work2, sl * n) || dalloc(&S->d_lim, sl * 3 * k))) return 1;
+    hipStream_t sm = C->stream;
+    HIPCHK(hipMemsetAsync(S->d_x, 0, sl * k * 4, sm));
+    HIPCHK(hipMemsetAsync(S->d_err, 0, sl * 4, sm));
+    HIPCHK(hipMemsetAsync(S->d_nbits, 0, sl * 4, sm));
+    HIPCHK(hipMemsetAsync(S->d_pos, 0, sl * 4, sm));
+    HIPCHK(hipMemsetAsync(S->d_exp, 0, sl * 4, sm));
+    // Constants: N'-hat by a weighted forward transform; A|B (or N-hat) from N.
+    {
+        uint32_t* limbs = nullptr;
+        if (dalloc(&limbs, sl * k)) return 1;
+        std::unique_ptr<uint32_t, void (*)(uint32_t*)> guard(limbs, [](uint32_t* p) { hipFree(p); });
+        const uint32_t tt = plan_threads(n);
+        HIPCHK(hipMemcpyAsync(limbs, st->nprime, sl * k * 4, hipMemcpyHostToDevice, sm));
+        hipLaunchKernelGGL(k_load_forward, dim3(st->slots), dim3(tt), 0, sm, S->tab->t, limbs, k, S->d_nphat1, S->d_nphat2);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(limbs, st->n, sl * k * 4, hipMemcpyHostToDevice, sm));
+        if (wrap) {
+            const uint32_t h = n / 2;
+            F1 c1 = cinv(F1{mod1(8 * h), 0});
+            F2 c2 = cinv(F2{(uint64_t)(8 * h), 0});
+            // The work of the half factors lives in the N'-hat-sized N-hat buffer's twin: use a
+            // temporary of 2h = n elements per slot.
+            F1* t1 = nullptr;
+            F2* t2 = nullptr;
+            if (dalloc(&t1, sl * n) || dalloc(&t2, sl * n)) { hipFree(t1); return 1; }
+            hipLaunchKernelGGL(k_half_factors, dim3(st->slots), dim3(plan_threads(h)), 0, sm, S->half->ht, limbs, k, t1, t2,
+                               S->d_nhat1, S->d_nhat2, c1, c2);
+            hipError_t le = hipGetLastError();
+            hipError_t se = hipStreamSynchronize(sm);
+            hipFree(t1);
+            hipFree(t2);
+            if (le != hipSuccess) return hip_fail("half factors launch", le);
+            if (se != hipSuccess) return hip_fail("half factors", se);
+        } else {
+            hipLaunchKernelGGL(k_load_forward, dim3(st->slots), dim3(tt), 0, sm, S->tab->t, limbs, k, S->d_nhat1, S->d_nhat2);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(sm));
+    }
+    S->h_x.assign(sl * k, 0);
+    S->exps.assign(sl, {});
+    S->nbits.assign(sl, 0);
+    S->pos.assign(sl, 0);
+    *out = S.release();
+    return 0;
+}
+
+void mm_destroy(void* st) { delete (State*)st; }
+
+int mm_load(void* st, size_t i, const uint32_t* limbs) {  // gpu/mod.rs:371-375
+    State* S = (State*)st;
+    if (i >= S->slots) return fail("slot out of range");
+    if (hipSetDevice(S->C->device) != hipSuccess || pull_x(S)) return fail("load: cannot read device values");
+    std::copy(limbs, limbs + S->k, S->h_x.begin() + i * S->k);
+    S->x_dirty = true;
+    return 0;
+}
+
+int mm_set_exponent(void* st, size_t i, const uint32_t* words, size_t nwords, uint64_t nbits) {  // gpu/mod.rs:376-382
+    State* S = (State*)st;
+    if (i >= S->slots) return fail("slot out of range");
+    if (nbits > 32ull * nwords || nbits > 0xffffffffull) return fail("nbits exceeds the exponent words");
+    S->exps[i].assign(words, words + nwords);
+    S->nbits[i] = (uint32_t)nbits;
+    S->pos[i] = 0;
+    S->exp_dirty = true;
+    return 0;
+}
+
+int64_t mm_run(void* st, uint64_t bits) { return steps((State*)st, bits, false); }
+int64_t mm_run_bare(void* st, uint64_t bits) { return steps((State*)st, bits, true); }
+
+int64_t mm_mul(void* st, size_t out, size_t a, size_t b) {  // gpu/mod.rs:386-391
+    State* S = (State*)st;
+    Ctx* C = S->C;
+    if (out >= S->slots || a >= S->slots || b >= S->slots) { fail("slot out of range"); return MM_EFAULT; }
+    if (hipSetDevice(C->device) != hipSuccess || push(S)) return MM_EFAULT;
+    const uint32_t n = S->n, k = S->k;
+    F1 *mb1 = nullptr, *w1 = nullptr;
+    F2 *mb2 = nullptr, *w2 = nullptr;
+    uint32_t* lim = nullptr;
+    int64_t r = MM_EFAULT;
+    if (!dalloc(&mb1, n) && !dalloc(&mb2, n) && !dalloc(&w1, n) && !dalloc(&w2, n) && !dalloc(&lim, 3ull * k)) {
+        auto kern = S->wrap ? k_mul<true> : k_mul<false>;
+        hipLaunchKernelGGL(kern, dim3(1), dim3(S->threads), lds_scratch_bytes(S->threads), C->stream, S->args(), (uint32_t)out,
+                           (uint32_t)a, (uint32_t)b, mb1, mb2, w1, w2, lim);
+        hipError_t le = hipGetLastError();
+        hipError_t se = hipStreamSynchronize(C->stream);
+        if (le != hipSuccess) hip_fail("mul launch", le);
+        else if (se != hipSuccess) hip_fail("mul", se);
+        else r = check_err(S, "mul");
+    }
+    for (void* p : {(void*)mb1, (void*)mb2, (void*)w1, (void*)w2, (void*)lim}) if (p) hipFree(p);
+    S->x_host_valid = false;
+    return r;
+}
+
+int mm_read(void* st, size_t i, uint32_t* out) {  // gpu/mod.rs:392-394
+    State* S = (State*)st;
+    if (i >= S->slots) return fail("slot out of range");
+    if (hipSetDevice(S->C->device) != hipSuccess || pull_x(S)) return 1;
+    std::copy(S->h_x.begin() + i * S->k, S->h_x.begin() + (i + 1) * S->k, out);
+    return 0;
+}
+
+uint64_t mm_remaining(void* st, size_t i) {  // gpu/mod.rs:395-397
+    State* S = (State*)st;
+    if (i >= S->slots) return 0;
+    return S->nbits[i] - S->pos[i];
+}
+
+int mm_push(void* st) {
+    State* S = (State*)st;
+    if (hipSetDevice(S->C->device) != hipSuccess) return fail("hipSetDevice failed");
+    if (push(S)) return 1;
+    HIPCHK(hipStreamSynchronize(S->C->stream));
+    return 0;
+}
+
+int mm_plan(void* st, mm_plan_info* out) {
+    State* S = (State*)st;
+    out->global = S->global;
+    out->threads = S->threads;
+    out->lds_bytes = (uint32_t)S->lds;
+    out->slots_per_wg = 1;
+    return 0;
+}
+
+uint64_t mm_free_memory(void* self) {
+    Ctx* C = (Ctx*)self;
+    size_t f = 0, t = 0;
+    hipSetDevice(C->device);
+    hipMemGetInfo(&f, &t);
+    return f;
+}
+int mm_cu_count(void* self) { return ((Ctx*)self)->cus; }
+uint64_t mm_max_lds(void* self) { return ((Ctx*)self)->max_lds; }
+int mm_device_name(void* self, char* buf, size_t len) {
+    Ctx* C = (Ctx*)self;
+    if (len) { strncpy(buf, C->name.c_str(), len - 1); buf[len - 1] = 0; }
+    return 0;
+}
+int mm_last_run_kernel_ms(void* st, double* ms, uint32_t* launches) {
+    State* S = (State*)st;
+    *ms = S->last_ms;
+    *launches = S->last_launches;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,338 @@
+// mm_host.cpp - the host API over any backend table: ModBatch, fermat, Params and the score
+// workload (montmul/src/batch.rs, montmul/src/score.rs:45-61).
+//
+// The reference's `ModBatch<B: Backend>` is generic over its backend; here the backend is an
+// `mm_backend` function table, so this one host drives the HIP backend of this library or any
+// other implementation of the table. Big-integer setup (N', R mod N, R^-1 mod N) uses GMP, as
+// the reference does through `rug` (batch.rs:84-94), one thread per modulus group.
+
+#include <gmp.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/montmul_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+int fail(const std::string& m) { g_err = m; return 1; }
+
+struct Mpz {
+    mpz_t v;
+    Mpz() { mpz_init(v); }
+    Mpz(const Mpz& o) { mpz_init_set(v, o.v); }
+    Mpz& operator=(const Mpz& o) { mpz_set(v, o.v); return *this; }
+    ~Mpz() { mpz_clear(v); }
+};
+
+void from_signed_limbs(mpz_t z, int sign, const uint32_t* l, size_t nl) {
+    mpz_import(z, nl, -1, 4, 0, 0, l);
+    if (sign < 0) mpz_neg(z, z);
+}
+
+// to_limbs (batch.rs:34-39): k little-endian limbs; fails when the value does not fit.
+bool to_limbs(const mpz_t z, uint32_t* out, size_t k) {
+    memset(out, 0, k * 4);
+    if (mpz_sgn(z) < 0 || mpz_sizeinbase(z, 2) > 32 * k) return false;
+    if (mpz_sgn(z) == 0) return true;
+    size_t cnt = 0;
+    mpz_export(out, &cnt, -1, 4, 0, 0, z);
+    return true;
+}
+
+size_t significant_bits(const mpz_t z) { return mpz_sgn(z) == 0 ? 0 : mpz_sizeinbase(z, 2); }
+
+template <class Fn> void par_for(size_t count, Fn fn) {
+    unsigned h = std::thread::hardware_concurrency();
+    unsigned nt = (unsigned)std::min<size_t>(h ? h : 1, count);
+    if (nt <= 1) { for (size_t i = 0; i < count; i++) fn(i); return; }
+    std::atomic<size_t> next(0);
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < nt; t++)
+        ts.emplace_back([&] { for (size_t i; (i = next.fetch_add(1)) < count;) fn(i); });
+    for (auto& t : ts) t.join();
+}
+
+typedef unsigned __int128 u128;
+
+}  // namespace
+
+// Modulus (batch.rs:44-50).
+struct MmModulus { Mpz n, r, rinv; };
+
+struct mm_modbatch {  // batch.rs:53-58
+    mm_backend be;
+    void* state = nullptr;
+    uint32_t k = 0, len = 0;
+    std::vector<MmModulus> moduli;
+    ~mm_modbatch() { if (state) be.destroy(state); }
+};
+
+extern "C" {
+
+const char* mm_host_last_error(void) { return g_err.c_str(); }
+
+// Params::for_bits (batch.rs:20-25).
+int mm_params_for_bits(uint32_t bits, uint32_t min_len, uint32_t max_len, uint32_t* k, uint32_t* len) {
+    uint64_t kk = ((uint64_t)bits + 4 + 31) / 32;
+    for (uint64_t l = min_len; l && l <= max_len; l *= 2)
+        if (l >= 2 * kk) { *k = (uint32_t)kk; *len = (uint32_t)l; return 0; }
+    return fail("no transform length for " + std::to_string(bits) + " bits");
+}
+
+// Params::check_exact (batch.rs:27-31): max(c, 1) * k * (2^32 - 1)^2 < bound, in exact
+// arithmetic (the left side can exceed 128 bits only for k far beyond any supported length).
+int mm_params_check_exact(uint64_t k, uint32_t c, uint64_t bound_lo, uint64_t bound_hi) {
+    Mpz lhs, bound, t;
+    mpz_set_ui(lhs.v, c ? c : 1);
+    mpz_mul_ui(lhs.v, lhs.v, k);
+    mpz_set_ui(t.v, 0xffffffffu);
+    mpz_mul(t.v, t.v, t.v);
+    mpz_mul(lhs.v, lhs.v, t.v);
+    mpz_set_ui(bound.v, bound_hi);
+    mpz_mul_2exp(bound.v, bound.v, 64);
+    mpz_add_ui(bound.v, bound.v, bound_lo);
+    if (mpz_cmp(lhs.v, bound.v) < 0) return 0;
+    char* a = mpz_get_str(nullptr, 10, lhs.v);
+    char* b = mpz_get_str(nullptr, 10, bound.v);
+    std::string m = std::string("c * k * (2^w - 1)^2 = ") + a + " exceeds the coefficient bound " + b;
+    void (*freefn)(void*, size_t);
+    mp_get_memory_functions(nullptr, nullptr, &freefn);
+    freefn(a, strlen(a) + 1);
+    freefn(b, strlen(b) + 1);
+    return fail(m);
+}
+
+// ModBatch::new (batch.rs:62-105).
+int mm_modbatch_new(const mm_backend* be, size_t count, const uint32_t* limbs, const size_t* offsets,
+                    const int8_t* signs, uint32_t base, mm_modbatch** out) {
+    *out = nullptr;
+    if (count == 0) return fail("empty batch");
+    if (base < 1 || base > 4)
+        return fail("base " + std::to_string(base) + " does not fit the lazy range (c <= 4)");
+    std::vector<Mpz> ns(count);
+    uint32_t k = 0, len = 0;
+    for (size_t i = 0; i < count; i++) {
+        from_signed_limbs(ns[i].v, signs[i], limbs + offsets[i], offsets[i + 1] - offsets[i]);
+        if (mpz_sgn(ns[i].v) <= 0 || mpz_even_p(ns[i].v) || mpz_cmp_ui(ns[i].v, 1) <= 0)
+            return fail("modulus must be odd and > 1");
+        size_t bits = significant_bits(ns[i].v);
+        if (bits > 0xffffffffu) return fail("modulus too large");
+        uint32_t pk, pl;
+        if (mm_params_for_bits((uint32_t)bits, be->min_len, be->max_len, &pk, &pl)) return 1;
+        if (i == 0) { k = pk; len = pl; continue; }
+        if (pl != len)
+            return fail("moduli need different transform lengths (" + std::to_string(len) + " vs " +
+                        std::to_string(pl) + ")");
+        k = std::max(k, pk);
+    }
+    if (be->full_k) k = len / 2;
+    if (mm_params_check_exact(k, base, be->bound_lo, be->bound_hi)) return 1;
+
+    auto B = new mm_modbatch();
+    B->be = *be; B->k = k; B->len = len;
+    B->moduli.resize(count);
+    std::vector<uint32_t> nl(count * (size_t)k), npl(count * (size_t)k), tnl(count * (size_t)k);
+    std::atomic<int> bad(0);
+    // Two modular inversions per modulus dominate the setup of large batches.
+    par_for(count, [&](size_t i) {
+        Mpz r, ninv, np, two;
+        mpz_setbit(r.v, 32 * (mp_bitcnt_t)k);
+        const mpz_t& n = ns[i].v;
+        if (!mpz_invert(ninv.v, n, r.v)) { bad = 1; return; }
+        mpz_sub(np.v, r.v, ninv.v);
+        mpz_mod(np.v, np.v, r.v);
+        MmModulus& m = B->moduli[i];
+        mpz_set(m.n.v, n);
+        mpz_mod(m.r.v, r.v, n);
+        if (!mpz_invert(m.rinv.v, r.v, n)) { bad = 1; return; }
+        mpz_mul_2exp(two.v, n, 1);
+        if (!to_limbs(n, &nl[i * k], k) || !to_limbs(np.v, &npl[i * k], k) || !to_limbs(two.v, &tnl[i * k], k))
+            bad = 1;
+    });
+    if (bad) { delete B; return fail("value does not fit " + std::to_string(k) + " limbs"); }
+    mm_batch_setup st{k, len, base, (uint32_t)count, nl.data(), npl.data(), tnl.data()};
+    if (be->create(be->self, &st, &B->state)) {
+        g_err = be->last_error();
+        B->state = nullptr;
+        delete B;
+        return 1;
+    }
+    *out = B;
+    return 0;
+}
+
+void mm_modbatch_free(mm_modbatch* b) { delete b; }
+
+void mm_modbatch_params(const mm_modbatch* b, uint32_t* k, uint32_t* len) { *k = b->k; *len = b->len; }
+size_t mm_modbatch_slots(const mm_modbatch* b) { return b->moduli.size(); }
+void* mm_modbatch_state(mm_modbatch* b) { return b->state; }
+
+static int check_slot(const mm_modbatch* b, size_t i) {
+    if (i >= b->moduli.size()) return fail("slot " + std::to_string(i) + " out of range");
+    return 0;
+}
+
+// ModBatch::load (batch.rs:125-130).
+int mm_modbatch_load(mm_modbatch* b, size_t i, int sign, const uint32_t* limbs, size_t nl) {
+    if (check_slot(b, i)) return 1;
+    const MmModulus& m = b->moduli[i];
+    Mpz x;
+    from_signed_limbs(x.v, sign, limbs, nl);
+    mpz_mul(x.v, x.v, m.r.v);
+    mpz_mod(x.v, x.v, m.n.v);
+    std::vector<uint32_t> l(b->k);
+    to_limbs(x.v, l.data(), b->k);
+    if (b->be.load(b->state, i, l.data())) return fail(b->be.last_error());
+    return 0;
+}
+
+// ModBatch::load_raw (batch.rs:133-137).
+int mm_modbatch_load_raw(mm_modbatch* b, size_t i, int sign, const uint32_t* limbs, size_t nl) {
+    if (check_slot(b, i)) return 1;
+    Mpz x, two;
+    from_signed_limbs(x.v, sign, limbs, nl);
+    mpz_mul_2exp(two.v, b->moduli[i].n.v, 1);
+    if (mpz_sgn(x.v) < 0 || mpz_cmp(x.v, two.v) >= 0) return fail("raw value outside [0, 2N)");
+    std::vector<uint32_t> l(b->k);
+    to_limbs(x.v, l.data(), b->k);
+    if (b->be.load(b->state, i, l.data())) return fail(b->be.last_error());
+    return 0;
+}
+
+// ModBatch::set_exponent (batch.rs:140-149): bit p of the words is bit nbits - 1 - p of e.
+int mm_modbatch_set_exponent(mm_modbatch* b, size_t i, int sign, const uint32_t* limbs, size_t nl) {
+    if (check_slot(b, i)) return 1;
+    if (sign < 0) return fail("exponent must be non-negative");
+    Mpz e;
+    from_signed_limbs(e.v, sign, limbs, nl);
+    size_t nbits = significant_bits(e.v);
+    std::vector<uint32_t> words((nbits + 31) / 32, 0);
+    for (size_t p = 0; p < nbits; p++)
+        if (mpz_tstbit(e.v, nbits - 1 - p)) words[p / 32] |= 1u << (31 - p % 32);
+    if (b->be.set_exponent(b->state, i, words.data(), words.size(), nbits)) return fail(b->be.last_error());
+    return 0;
+}
+
+int64_t mm_modbatch_run(mm_modbatch* b, uint64_t bits) {  // batch.rs:152-154
+    int64_t r = b->be.run(b->state, bits);
+    if (r != MM_OK) g_err = b->be.last_error();
+    return r;
+}
+
+uint64_t mm_modbatch_remaining(mm_modbatch* b, size_t i) {  // batch.rs:157-159
+    if (check_slot(b, i)) return 0;
+    return b->be.remaining(b->state, i);
+}
+
+int64_t mm_modbatch_mul(mm_modbatch* b, size_t out, size_t a, size_t c) {  // batch.rs:162-166
+    if (check_slot(b, out) || check_slot(b, a) || check_slot(b, c)) return MM_EFAULT;
+    if (mpz_cmp(b->moduli[out].n.v, b->moduli[a].n.v) || mpz_cmp(b->moduli[a].n.v, b->moduli[c].n.v)) {
+        fail("mul: slots must share a modulus");
+        return MM_EFAULT;
+    }
+    int64_t r = b->be.mul(b->state, out, a, c);
+    if (r != MM_OK) g_err = b->be.last_error();
+    return r;
+}
+
+int mm_modbatch_read_raw(mm_modbatch* b, size_t i, uint32_t* out) {  // batch.rs:168-170
+    if (check_slot(b, i)) return 1;
+    if (b->be.read(b->state, i, out)) return fail(b->be.last_error());
+    return 0;
+}
+
+int mm_modbatch_read(mm_modbatch* b, size_t i, uint32_t* out) {  // batch.rs:173-176
+    if (mm_modbatch_read_raw(b, i, out)) return 1;
+    const MmModulus& m = b->moduli[i];
+    Mpz x;
+    mpz_import(x.v, b->k, -1, 4, 0, 0, out);
+    mpz_mul(x.v, x.v, m.rinv.v);
+    mpz_mod(x.v, x.v, m.n.v);
+    to_limbs(x.v, out, b->k);
+    return 0;
+}
+
+int mm_modbatch_is_one(mm_modbatch* b, size_t i) {  // batch.rs:179-183
+    std::vector<uint32_t> l(b->k);
+    if (mm_modbatch_read_raw(b, i, l.data())) return -1;
+    const MmModulus& m = b->moduli[i];
+    Mpz v, r2;
+    mpz_import(v.v, b->k, -1, 4, 0, 0, l.data());
+    mpz_add(r2.v, m.r.v, m.n.v);
+    return mpz_cmp(v.v, m.r.v) == 0 || mpz_cmp(v.v, r2.v) == 0;
+}
+
+int mm_modbatch_modulus(mm_modbatch* b, size_t i, uint32_t* out) {
+    if (check_slot(b, i)) return 1;
+    to_limbs(b->moduli[i].n.v, out, b->k);
+    return 0;
+}
+
+// fermat (batch.rs:187-198).
+int mm_fermat(const mm_backend* be, size_t count, const uint32_t* limbs, const size_t* offsets,
+              const int8_t* signs, uint32_t base, uint64_t chunk, uint8_t* prp) {
+    if (chunk == 0) return fail("chunk must be positive");
+    mm_modbatch* b;
+    if (mm_modbatch_new(be, count, limbs, offsets, signs, base, &b)) return 1;
+    uint32_t one = 1;
+    for (size_t i = 0; i < count; i++) {
+        Mpz e;
+        mpz_set(e.v, b->moduli[i].n.v);
+        mpz_sub_ui(e.v, e.v, 1);
+        std::vector<uint32_t> el((significant_bits(e.v) + 31) / 32 + 1, 0);
+        size_t cnt = 0;
+        if (mpz_sgn(e.v)) mpz_export(el.data(), &cnt, -1, 4, 0, 0, e.v);
+        if (mm_modbatch_load(b, i, 1, &one, 1) || mm_modbatch_set_exponent(b, i, 1, el.data(), el.size())) {
+            mm_modbatch_free(b);
+            return 1;
+        }
+    }
+    for (;;) {
+        bool any = false;
+        for (size_t i = 0; i < count && !any; i++) any = b->be.remaining(b->state, i) > 0;
+        if (!any) break;
+        int64_t r = mm_modbatch_run(b, chunk);
+        if (r != MM_OK) { mm_modbatch_free(b); return 1; }
+    }
+    for (size_t i = 0; i < count; i++) {
+        int v = mm_modbatch_is_one(b, i);
+        if (v < 0) { mm_modbatch_free(b); return 1; }
+        prp[i] = (uint8_t)v;
+    }
+    mm_modbatch_free(b);
+    return 0;
+}
+
+// The score workload (score.rs:45-61): `slots` moduli of `bits` bits from GMP's default
+// random state seeded with `seed` (rug's RandState::new + seed + Integer::random_bits), top bit
+// and bit 0 set, and per modulus the top `ebits` bits of N - 1, i.e. (N - 1) >> (bits - ebits).
+// Writes ceil(bits/32) limbs per modulus and ceil(ebits/32) limbs per exponent, little-endian.
+int mm_score_workload(uint32_t bits, size_t slots, uint32_t ebits, uint64_t seed, uint32_t* moduli, uint32_t* exps) {
+    if (slots == 0 || bits < 2 || ebits == 0 || ebits > bits) return fail("slots > 0 and 0 < ebits <= bits required");
+    gmp_randstate_t rs;
+    gmp_randinit_default(rs);
+    Mpz s, n, e;
+    mpz_set_ui(s.v, (unsigned long)seed);
+    gmp_randseed(rs, s.v);
+    size_t nl = (bits + 31) / 32, el = (ebits + 31) / 32;
+    for (size_t i = 0; i < slots; i++) {
+        mpz_urandomb(n.v, rs, bits);
+        mpz_setbit(n.v, bits - 1);
+        mpz_setbit(n.v, 0);
+        to_limbs(n.v, moduli + i * nl, nl);
+        mpz_sub_ui(e.v, n.v, 1);
+        mpz_fdiv_q_2exp(e.v, e.v, bits - ebits);
+        to_limbs(e.v, exps + i * el, el);
+    }
+    gmp_randclear(rs);
+    return 0;
+}
+
+}  // extern "C"

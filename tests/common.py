@@ -1,0 +1,278 @@
+"""Backend-generic correctness cases, restating the reference's shared test library
+(montmul/tests/common/mod.rs). Every check compares against Python integers (pow, modular
+inverse, exact products), which play the role of GMP through `rug` in the reference.
+
+`make` is a callable returning a fresh backend (an object with a `table` and the Limits
+methods); `ModBatch` and `fermat` are the host API under test.
+"""
+
+import random
+
+from montmul_amd import MontmulError, ModBatch, Params, fermat
+
+
+def rand_odd(rng, bits):
+    """Random odd integer of exactly `bits` bits (common/mod.rs:10-15)."""
+    return rng.getrandbits(bits) | (1 << (bits - 1)) | 1
+
+
+def expected(x, c, n, k):
+    """x^2 c R^-1 mod N, the expected step result (common/mod.rs:17-21)."""
+    rinv = pow(1 << (32 * k), -1, n)
+    return x * x * c * rinv % n
+
+
+def tribonacci(n):
+    """T(n), T(0) = T(1) = 0, T(2) = 1 (OEIS A000073; seq.rs:6-15)."""
+    a, b, c = 0, 0, 1
+    for _ in range(n):
+        a, b, c = b, c, a + b + c
+    return a
+
+
+# OEIS A092835: n - 1 for prime T(n) (common/mod.rs:284-288).
+A092835_SMALL = [6, 7, 11, 87, 98, 215, 802]
+A092835_LARGE = [4202, 18699, 96879]
+
+
+def random_modmul(make, bits, cases, seed, base):
+    """Random odd N, random x in [0, 2N); one step with c = base, then one with c = 1
+    (common/mod.rs:93-120)."""
+    rng = random.Random(seed)
+    moduli = [rand_odd(rng, bits) for _ in range(cases)]
+    batch = ModBatch(make(), moduli, base)
+    k = batch.params().k
+    xs = [rng.randrange(2 * n) for n in moduli]
+    for i, x in enumerate(xs):
+        batch.load_raw(i, x)
+        batch.set_exponent(i, 2)
+    prev = list(xs)
+    for c in (base, 1):
+        batch.run(1)
+        for i, n in enumerate(moduli):
+            raw = batch.read_raw(i)
+            assert raw < 2 * n, f"lazy range, bits {bits} slot {i}"
+            assert raw % n == expected(prev[i], c, n, k), f"bits {bits} slot {i} c {c}"
+            prev[i] = raw
+
+
+def edge_sizes(max_len_log2=16):
+    sizes = [28, 60, 124, 252, 1020, 4092, 32764]
+    for m in range(3, max_len_log2 + 1):
+        kmax = (1 << m) // 2
+        sizes.append(32 * kmax - 4)           # top of the bucket
+        sizes.append(32 * (kmax // 2) - 3)    # bottom of the bucket
+    return sizes
+
+
+def edges(make, base, max_len_log2=16, seed=1):
+    """x = 0, N, 2N - 1 with N random and N = 2^b - 1, for bucket edges (common/mod.rs:122-159)."""
+    rng = random.Random(seed)
+    for bits in edge_sizes(max_len_log2):
+        if bits < 2:
+            continue
+        moduli = [rand_odd(rng, bits), (1 << bits) - 1]
+        batch = ModBatch(make(), moduli, base)
+        k = batch.params().k
+        for which in range(3):
+            for i, n in enumerate(moduli):
+                x = [0, n, 2 * n - 1][which]
+                batch.load_raw(i, x)
+                batch.set_exponent(i, 1)
+            batch.run(1)
+            for i, n in enumerate(moduli):
+                x = [0, n, 2 * n - 1][which]
+                raw = batch.read_raw(i)
+                assert raw < 2 * n, f"lazy range bits {bits}"
+                assert raw % n == expected(x, base, n, k), f"bits {bits} case {which}"
+
+
+def batch_independence(make, bits, sizes, seed=2):
+    """One modulus gives identical raw limbs in every slot position and batch size
+    (common/mod.rs:161-188)."""
+    rng = random.Random(seed)
+    target = rand_odd(rng, bits)
+    e = rng.getrandbits(64) | 1
+    reference = None
+    for size in sizes:
+        moduli = [rand_odd(rng, bits) for _ in range(size)]
+        pos = min(size * 5 // 7, size - 1)
+        for p in (0, pos, size - 1):
+            moduli[p] = target
+        batch = ModBatch(make(), moduli, 3)
+        for i in range(size):
+            batch.load(i, 1)
+            batch.set_exponent(i, e)
+        batch.run(1000)
+        for p in (0, pos, size - 1):
+            raw = batch.read_raw(p)
+            if reference is None:
+                reference = raw
+            assert raw == reference, f"size {size} slot {p}"
+            assert batch.read(p) == pow(3, e, target)
+
+
+def powmod_and_mul(make, bits, count, seed=3):
+    """Chunked exponentiation against pow, then `mul` (common/mod.rs:190-220)."""
+    rng = random.Random(seed)
+    moduli = [rand_odd(rng, bits) for _ in range(count)]
+    batch = ModBatch(make(), moduli, 3)
+    exps = []
+    for i in range(count):
+        e = rng.getrandbits(32 + i * 7) | 1
+        batch.load(i, 1)
+        batch.set_exponent(i, e)
+        exps.append(e)
+    while any(batch.remaining(i) > 0 for i in range(count)):
+        batch.run(37)
+    for i, n in enumerate(moduli):
+        assert batch.read(i) == pow(3, exps[i], n), f"slot {i}"
+    n = rand_odd(rng, bits)
+    batch = ModBatch(make(), [n, n, n], 3)
+    a, b = rng.randrange(n), rng.randrange(n)
+    batch.load(0, a)
+    batch.load(1, b)
+    batch.mul(2, 0, 1)
+    assert batch.read(2) == a * b % n
+    batch.mul(0, 0, 0)
+    assert batch.read(0) == a * a % n
+
+
+def rand_exponent(rng, bits):
+    return rng.getrandbits(bits) | (1 << (bits - 1))
+
+
+def assert_same_raw(make_a, make_b, bits, slots, steps, seed=4):
+    """Two backends on the same moduli and exponents end with equal raw limbs
+    (common/mod.rs:229-247): the parity primitive."""
+    rng = random.Random(seed)
+    moduli = [rand_odd(rng, bits) for _ in range(slots)]
+    exps = [rand_exponent(rng, steps) for _ in range(slots)]
+    x = ModBatch(make_a(), moduli, 3)
+    y = ModBatch(make_b(), moduli, 3)
+    for i, e in enumerate(exps):
+        for b in (x, y):
+            b.load(i, 1)
+            b.set_exponent(i, e)
+    x.run(steps)
+    y.run(steps)
+    for i in range(slots):
+        assert x.read_raw(i) == y.read_raw(i), f"bits {bits} slot {i}"
+
+
+def long_run(make, bits, slots, ebits, seed=5):
+    """One run(ebits) from load(1) gives 3^e mod N (common/mod.rs:249-264)."""
+    rng = random.Random(seed)
+    moduli = [rand_odd(rng, bits) for _ in range(slots)]
+    exps = [rand_exponent(rng, ebits) for _ in range(slots)]
+    batch = ModBatch(make(), moduli, 3)
+    for i, e in enumerate(exps):
+        batch.load(i, 1)
+        batch.set_exponent(i, e)
+    batch.run(ebits)
+    for i, n in enumerate(moduli):
+        assert batch.remaining(i) == 0, f"slot {i}"
+        assert batch.read(i) == pow(3, exps[i], n), f"bits {bits} slot {i}"
+
+
+def end_to_end(make, ns):
+    """Fermat base 3 on T(n), grouped by transform length (common/mod.rs:266-282)."""
+    moduli = [t for t in (tribonacci(n) for n in ns) if t % 2 == 1 and t > 3]
+    lengths = make().supported_lengths()
+    groups = {}
+    for m in moduli:
+        groups.setdefault(Params.for_bits(m.bit_length(), lengths).len, []).append(m)
+    for ms in groups.values():
+        got = fermat(make(), ms, 3, 500)
+        for m, g in zip(ms, got):
+            assert g == (pow(3, m - 1, m) == 1), f"T with {m.bit_length()} bits"
+
+
+def known_prps(make, prime_ns, composite_ns):
+    """Known tribonacci primes report PRP, composites do not (common/mod.rs:290-300)."""
+    for n, want in [(p, True) for p in prime_ns] + [(c, False) for c in composite_ns]:
+        t = tribonacci(n)
+        if t % 2 == 0:
+            continue
+        assert fermat(make(), [t], 3, 1 << 20)[0] == want, f"T({n})"
+
+
+def api_contract(make, seed=6):
+    """Parameter rules, rejection of bad input, canonical read, is_one, chunked runs,
+    frozen slots and re-arming (common/mod.rs:302-374)."""
+    import pytest
+
+    lengths = make().supported_lengths()
+    assert all(l & (l - 1) == 0 for l in lengths)
+    for bad, base, what in [([], 3, "empty batch"), ([10], 3, "even modulus"), ([1], 3, "modulus 1"),
+                            ([-7], 3, "negative modulus"), ([11], 5, "base above lazy range"),
+                            ([11], 0, "base 0"), ([101, (1 << 5000) + 1], 3, "mixed transform lengths")]:
+        with pytest.raises(MontmulError):
+            ModBatch(make(), bad, base)
+    for bits in [2, 28, 29, 60, 61, 1000, 87_914]:
+        p = Params.for_bits(bits, lengths)
+        assert p.k * 32 >= bits + 4 and (p.k - 1) * 32 < bits + 4
+        assert p.len >= 2 * p.k and (p.len == lengths[0] or p.len // 2 < 2 * p.k)
+    if make().coefficient_bound() < 2**128 - 1:
+        with pytest.raises(MontmulError):
+            Params(1 << 30, 1 << 31).check_exact(4, make().coefficient_bound())
+    rng = random.Random(seed)
+    for n in [3, 5, 7, 9, 15, 101, 65_537, 4_294_967_291, 2**32 - 1]:
+        b = ModBatch(make(), [n], 3)
+        for x in [0, 1, n, n * 5 + 2, -4]:
+            b.load(0, x)
+            assert b.read(0) == x % n, f"read after load, n {n}"
+            assert b.is_one(0) == (x % n == 1), f"is_one, n {n}"
+            e = rng.getrandbits(40) + 1
+            b.load(0, x)
+            b.set_exponent(0, e)
+            b.run(1 << 20)
+            assert b.remaining(0) == 0
+            sq = 1 << e.bit_length()
+            want = pow(x % n, sq, n) * pow(3, e, n) % n
+            assert b.read(0) == want, f"pow, n {n}"
+    moduli = [rand_odd(rng, 777) for _ in range(5)]
+    exps = [rng.getrandbits(10 + 60 * i) | 1 for i in range(5)]
+    a = ModBatch(make(), moduli, 3)
+    b = ModBatch(make(), moduli, 3)
+    for i in range(5):
+        for m in (a, b):
+            m.load(i, 1)
+            m.set_exponent(i, exps[i])
+    a.run(0)
+    assert a.remaining(4) == exps[4].bit_length()
+    a.run(1000)
+    for chunk in [1, 2, 3, 5, 8, 13, 21, 34, 55, 89, 144, 1000]:
+        b.run(chunk)
+    for i in range(5):
+        assert a.read(i) == pow(3, exps[i], moduli[i])
+        assert a.read_raw(i) == b.read_raw(i), f"chunked run slot {i}"
+    a.run(100)
+    assert a.read(0) == pow(3, exps[0], moduli[0]), "frozen after finish"
+    a.load(0, 2)
+    a.set_exponent(0, 5)
+    a.run(10)
+    assert a.read(0) == (256 * 243) % moduli[0]
+
+
+def carry_stress(make, max_len):
+    """x = 2N - 1 with N = 2^b - 1 and N = 2^b - 2^(b/2) + 1, 32 squarings, every bucket
+    (common/mod.rs:376-398)."""
+    length = 8
+    while length <= max_len:
+        bits = 32 * (length // 2) - 4
+        n = (1 << bits) - 1
+        n2 = (1 << bits) - (1 << (bits // 2)) + 1
+        b = ModBatch(make(), [n, n2], 3)
+        e = 0xFFFFFFFF
+        for i, m in enumerate((n, n2)):
+            b.load_raw(i, 2 * m - 1)
+            b.set_exponent(i, e)
+        b.run(32)
+        k = b.params().k
+        for i, m in enumerate((n, n2)):
+            rinv = pow(1 << (32 * k), -1, m)
+            x = (m - rinv) % m
+            want = pow(x, 1 << 32, m) * pow(3, e, m) % m
+            assert b.read(i) == want, f"len {length} slot {i}"
+        length *= 2

@@ -1,0 +1,274 @@
+"""GPU parity: the HIP backend against the oracle and the closed form, through the C ABI.
+
+Mirrors montmul/tests/gpu.rs at sizes the oracle finishes in seconds. Every comparison is bit
+for bit on raw Montgomery limbs (all integer arithmetic: no tolerance applies).
+"""
+
+import json
+import os
+import random
+
+import pytest
+
+import common
+import montmul_amd as mm
+from montmul_amd import CarryError, Hip, ModBatch, MontmulError
+from oracle_backends import Cpu, Gmp
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def hip():
+    h = Hip()
+    yield h
+    h.close()
+
+
+@pytest.fixture(scope="module")
+def hip_global():
+    h = Hip(force_global=True)
+    yield h
+    h.close()
+
+
+def test_device(hip):
+    say("device", hip.device_name(), "CUs", hip.cu_count(), "LDS", hip.max_lds())
+    assert Hip.device_count() >= 1
+    assert hip.cu_count() > 0 and hip.max_lds() >= 65536
+    assert hip.full_k()
+
+
+@pytest.mark.parametrize("mode", ["lds", "global"])
+def test_golden_steps(hip, hip_global, mode):
+    """Every golden step vector (tests/golden/steps.json), one step each."""
+    be = hip if mode == "lds" else hip_global
+    with open(os.path.join(GOLD, "steps.json")) as f:
+        steps = json.load(f)
+    groups = {}
+    for v in steps:
+        groups.setdefault((v["L"], v["c"]), []).append(v)
+    for (L, c), vs in sorted(groups.items()):
+        say("golden", mode, "L", L, "c", c, "vectors", len(vs))
+        b = ModBatch(be, [int(v["n"], 16) for v in vs], max(c, 1))
+        assert b.params().k == vs[0]["k"] and b.params().len == L
+        for i, v in enumerate(vs):
+            b.load_raw(i, int(v["x"], 16))
+            b.set_exponent(i, 1)
+        b.run(1)
+        for i, v in enumerate(vs):
+            assert b.read_raw(i) == int(v["out"], 16), f"L {L} c {c} bits {v['bits']} vector {i}"
+
+
+def test_golden_chains(hip):
+    with open(os.path.join(GOLD, "chains.json")) as f:
+        chains = json.load(f)
+    for ch in chains:
+        say("chain bits", ch["bits"])
+        b = ModBatch(hip, [int(ch["n"], 16)], 3)
+        b.load(0, 1)
+        b.set_exponent(0, int(ch["e"], 16))
+        for raw in ch["raw"]:
+            b.run(1)
+            assert b.read_raw(0) == int(raw, 16)
+        assert b.read(0) == int(ch["pow"], 16)
+
+
+def test_gpu_matches_cpu_raw(hip):
+    """GPU (wraparound, k = L/2) against the Cpu restatement (tests/gpu.rs:169-177, 315-330)."""
+    for bits, slots, steps in [(64, 8, 60), (200, 8, 60), (1000, 8, 60), (5000, 8, 40), (87_914, 4, 12),
+                               (100_000, 4, 12)]:
+        say("same raw", bits, slots, steps)
+        common.assert_same_raw(lambda: hip, lambda: Cpu(wraparound=True), bits, slots, steps)
+
+
+def test_global_mode_matches_lds_mode(hip, hip_global):
+    """tests/gpu.rs:155-167: same raw limbs in both modes, and the modes are as planned."""
+    n = common.rand_odd(random.Random(11), 87_914)
+    assert not Hip.plan_of(ModBatch(hip, [n], 3))["global"]
+    assert Hip.plan_of(ModBatch(hip_global, [n], 3))["global"]
+    say("global == lds at 87,914 bits")
+    common.assert_same_raw(lambda: hip, lambda: hip_global, 87_914, 16, 60)
+    common.assert_same_raw(lambda: hip, lambda: hip_global, 200, 8, 60)
+
+
+def test_large_lengths_global(hip):
+    """L = 16,384 and 32,768 do not fit LDS and run in global mode (tests/gpu.rs:325-329)."""
+    for bits in (200_000, 400_000):
+        b = ModBatch(hip, [common.rand_odd(random.Random(bits), bits)], 3)
+        assert Hip.plan_of(b)["global"]
+        say("large L", b.params().len)
+        common.assert_same_raw(lambda: hip, lambda: Cpu(wraparound=True), bits, 2, 6)
+
+
+def test_random_modmul(hip):
+    for bits, cases in [(64, 100), (1000, 60), (10_000, 16), (100_000, 4)]:
+        for base in (3, 4):
+            say("random_modmul", bits, base)
+            common.random_modmul(lambda: hip, bits, cases, bits, base)
+
+
+def test_edges(hip):
+    for base in (3, 4):
+        say("edges", base)
+        common.edges(lambda: hip, base, max_len_log2=15)
+
+
+def test_independence(hip):
+    common.batch_independence(lambda: hip, 3000, [1, 7, 64, 1024])
+
+
+def test_powmod_mul(hip):
+    common.powmod_and_mul(lambda: hip, 2000, 9)
+    common.powmod_and_mul(lambda: hip, 87_914, 2)
+
+
+def test_api(hip):
+    common.api_contract(lambda: hip)
+
+
+def test_carries(hip):
+    for length in (512, 2048, 8192, 65_536):
+        say("carry stress L", length)
+        common.carry_stress(lambda: hip, length)
+
+
+def test_fermat_tribonacci(hip):
+    common.end_to_end(lambda: hip, [6, 7, 11, 87, 98, 215, 802, 1001, 2000, 4000])
+
+
+def test_tribonacci_prps(hip):
+    common.known_prps(lambda: hip, [4202, 18699], [4203, 18700])
+
+
+def carry_overflow(make, bits):
+    """x = R - 1 is outside [0, 2N); T = 4 (R - 1)^2 carries out of 2k limbs and the error names
+    slot 1 (tests/gpu.rs:269-305)."""
+    rng = random.Random(12)
+    moduli = [common.rand_odd(rng, bits), common.rand_odd(rng, bits)]
+    b = ModBatch(make(), moduli, 4)
+    k = b.params().k
+    for i in range(2):
+        b.load(i, 1)
+        b.set_exponent(i, 1)
+    import ctypes
+    buf = (ctypes.c_uint32 * k)(*([0xFFFFFFFF] * k))
+    assert mm.lib.mm_load(ctypes.c_void_p(b.state), 1, buf) == 0
+    with pytest.raises(CarryError, match="slot 1") as ei:
+        b.run(1)
+    assert ei.value.slot == 1
+
+
+def test_carry_overflow_names_the_slot(hip, hip_global):
+    carry_overflow(lambda: hip, 1000)
+    carry_overflow(lambda: hip, 87_914)
+    carry_overflow(lambda: hip_global, 87_914)
+
+
+def test_without_wraparound():
+    """tests/gpu.rs:332-345 at smaller sizes."""
+    h = Hip(wraparound=False)
+    hg = Hip(wraparound=False, force_global=True)
+    try:
+        assert not h.full_k()
+        common.assert_same_raw(lambda: h, Cpu, 100_000, 4, 12)
+        common.assert_same_raw(lambda: hg, Cpu, 87_914, 4, 12)
+        common.powmod_and_mul(lambda: h, 2000, 9)
+        common.random_modmul(lambda: h, 1000, 40, 1000, 3)
+        common.edges(lambda: h, 3, max_len_log2=13)
+        common.carry_stress(lambda: h, 1 << 13)
+    finally:
+        h.close()
+        hg.close()
+
+
+def test_long_run_split_into_launches():
+    """Runs split into launches of about max_launch_ms (tests/gpu.rs:179-184, shorter)."""
+    h = Hip(max_launch_ms=20)
+    try:
+        common.long_run(lambda: h, 87_914, 8, 600)
+    finally:
+        h.close()
+
+
+def test_two_batches_interleaved(hip):
+    """Two live batches with different L, run alternately (tests/gpu.rs:227-258)."""
+    rng = random.Random(13)
+    batches = []
+    for bits in (3000, 50_000):
+        moduli = [common.rand_odd(rng, bits) for _ in range(4)]
+        exps = [rng.getrandbits(300) | 1 for _ in range(4)]
+        b = ModBatch(hip, moduli, 3)
+        for i, e in enumerate(exps):
+            b.load(i, 1)
+            b.set_exponent(i, e)
+        batches.append((b, moduli, exps))
+    assert batches[0][0].params().len != batches[1][0].params().len
+    while any(b.remaining(i) > 0 for b, _, _ in batches for i in range(4)):
+        for b, _, _ in batches:
+            b.run(7)
+    for b, moduli, exps in batches:
+        for i in range(4):
+            assert b.read(i) == pow(3, exps[i], moduli[i]), f"slot {i}"
+
+
+def test_oversized_batch_is_error(hip):
+    """create fails with a message naming bytes before allocating (tests/gpu.rs:186-196). The
+    setup is passed straight to the backend: its limb arrays are not read before the memory
+    check, and 2^31 slots at L = 8,192 need far more than any device holds."""
+    import ctypes
+    dummy = (ctypes.c_uint32 * 4096)()
+    st = mm.BatchSetup(4096, 8192, 3, 2**31, dummy, dummy, dummy)
+    out = ctypes.c_void_p()
+    rc = hip.table.create(hip.table.self, ctypes.byref(st), ctypes.byref(out))
+    assert rc != 0 and not out.value
+    assert "bytes" in hip.table.last_error().decode()
+
+
+def test_no_leak_on_drop(hip):
+    rng = random.Random(14)
+    moduli = [common.rand_odd(rng, 50_000) for _ in range(64)]
+
+    def once():
+        b = ModBatch(hip, moduli, 3)
+        for i in range(len(moduli)):
+            b.load(i, 1)
+            b.set_exponent(i, 12345)
+        b.run(5)
+        b.mul(0, 0, 0)
+        b.free()
+
+    once()
+    before = hip.free_memory()
+    for _ in range(20):
+        once()
+    after = hip.free_memory()
+    assert after + (16 << 20) >= before, f"free device memory {before} -> {after}"
+
+
+def test_headline_shape_closed_form(hip, hip_global):
+    """100,000-bit moduli (L = 8,192, k = 4,096), 256 slots: every slot's raw limbs equal the
+    closed-form REDC chain with k = L/2 (GMP), in both modes."""
+    moduli, exps = mm.score_workload(100_000, 256, 16, 2026)
+    ref = ModBatch(Gmp(full_k=True), moduli, 3)
+    for i, e in enumerate(exps):
+        ref.load(i, 1)
+        ref.set_exponent(i, e)
+    ref.run(16)
+    for be, name in ((hip, "lds"), (hip_global, "global")):
+        say("headline shape", name)
+        b = ModBatch(be, moduli, 3)
+        assert b.params().k == 4096 and b.params().len == 8192
+        for i, e in enumerate(exps):
+            b.load(i, 1)
+            b.set_exponent(i, e)
+        b.run(16)
+        bad = [i for i in range(256) if b.read_raw(i) != ref.read_raw(i)]
+        assert not bad, f"{name}: slots {bad[:8]} differ"
+        b.free()

@@ -1,0 +1,97 @@
+"""The host API (ModBatch, fermat, Params; montmul/src/batch.rs) on CPU, driven over the
+oracle's backend tables: the shared cases of montmul/tests/cpu.rs and tests/gmp.rs at sizes
+that finish in seconds."""
+
+import pytest
+
+import common
+from montmul_amd import MontmulError, ModBatch, fermat
+from oracle_backends import Cpu, Gmp
+
+BACKENDS = {
+    "cpu": Cpu,
+    "cpu_wrap": lambda: Cpu(wraparound=True),
+    "gmp": Gmp,
+}
+
+
+@pytest.mark.parametrize("name", list(BACKENDS))
+def test_random_modmul(name):
+    make = BACKENDS[name]
+    for bits, cases in [(64, 60), (100, 60), (1000, 30), (10_000, 6)]:
+        for base in (3, 4):
+            common.random_modmul(make, bits, cases, bits, base)
+
+
+@pytest.mark.parametrize("name", list(BACKENDS))
+def test_edges(name):
+    for base in (3, 4):
+        common.edges(BACKENDS[name], base, max_len_log2=11)
+
+
+@pytest.mark.parametrize("name", list(BACKENDS))
+def test_independence(name):
+    common.batch_independence(BACKENDS[name], 3000, [1, 7, 16])
+
+
+@pytest.mark.parametrize("name", list(BACKENDS))
+def test_powmod_mul(name):
+    common.powmod_and_mul(BACKENDS[name], 2000, 9)
+
+
+@pytest.mark.parametrize("name", list(BACKENDS))
+def test_api(name):
+    common.api_contract(BACKENDS[name])
+
+
+@pytest.mark.parametrize("name", list(BACKENDS))
+def test_carries(name):
+    common.carry_stress(BACKENDS[name], 1 << 11)
+
+
+@pytest.mark.parametrize("name", list(BACKENDS))
+def test_fermat_tribonacci(name):
+    common.end_to_end(BACKENDS[name], common.A092835_SMALL + [100, 200, 1001])
+
+
+def test_known_prps():
+    common.known_prps(Gmp, common.A092835_SMALL, [8, 9, 10, 12, 13, 20, 86, 88, 803])
+
+
+def test_is_one_accepts_upper_lazy_representative():  # tests/cpu.rs:83-95
+    n = common.rand_odd(__import__("random").Random(1), 1000)
+    b = ModBatch(Cpu(), [n], 3)
+    r = (1 << (32 * b.params().k)) % n
+    b.load_raw(0, r + n)
+    assert b.is_one(0)
+    b.load_raw(0, r)
+    assert b.is_one(0)
+    b.load_raw(0, r + 1)
+    assert not b.is_one(0)
+
+
+def test_load_raw_rejects_out_of_range():  # tests/cpu.rs:97-103
+    n = 1_000_003
+    b = ModBatch(Cpu(), [n], 3)
+    with pytest.raises(MontmulError, match=r"raw value outside \[0, 2N\)"):
+        b.load_raw(0, 2 * n)
+
+
+def test_fermat_chunk_zero_is_error():  # tests/cpu.rs:105-109
+    with pytest.raises(MontmulError):
+        fermat(Cpu(), [1_000_003], 3, 0)
+
+
+def test_long_run():
+    common.long_run(lambda: Cpu(wraparound=True), 3000, 4, 300)
+
+
+def test_params_rules():
+    from montmul_amd import Params
+    lengths = Cpu().supported_lengths()
+    assert lengths[0] == 8 and lengths[-1] == 1 << 24
+    assert Params.for_bits(100_000, lengths) == Params(3126, 8192)
+    assert Params.for_bits(20_000, lengths) == Params(626, 2048)
+    assert Params.for_bits(4096, lengths) == Params(129, 512)
+    b = ModBatch(Cpu(wraparound=True), [(1 << 99_999) | 1], 3)
+    assert b.params() == Params(4096, 8192)
