@@ -319,6 +319,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     int cus = 0;
     size_t max_lds = 0;
+    size_t max_dyn_lds = 0;  // dynamic LDS every step kernel can take
     std::string name;
     std::mutex mu;
     std::map<uint32_t, std::shared_ptr<DevTables>> tabs;
@@ -436,12 +437,24 @@ StepKernel pick_step(bool global, bool wrap, bool bare) {
     return wrap ? k_steps<false, true, false> : k_steps<false, false, false>;
 }
 
+// Raises each step kernel's dynamic LDS limit to what the device allows beside the kernel's
+// static LDS, and records the smallest such limit for planning.
 int set_lds_limit(Ctx* C) {
     const StepKernel ks[] = {k_steps<true, true, true>,   k_steps<true, false, true>,  k_steps<true, true, false>,
                              k_steps<true, false, false>, k_steps<false, true, true>,  k_steps<false, false, true>,
                              k_steps<false, true, false>, k_steps<false, false, false>};
-    for (StepKernel k : ks)
-        HIPCHK(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C->max_lds));
+    size_t dyn_max = C->max_lds;
+    for (StepKernel k : ks) {
+        hipFuncAttributes a;
+        HIPCHK(hipFuncGetAttributes(&a, (const void*)k));
+        size_t dyn = C->max_lds > a.sharedSizeBytes ? C->max_lds - a.sharedSizeBytes : 0;
+        if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) {
+            (void)hipGetLastError();
+            dyn = std::min<size_t>(dyn, 65536);
+        }
+        dyn_max = std::min(dyn_max, dyn);
+    }
+    C->max_dyn_lds = dyn_max;
     return 0;
 }
 
@@ -626,7 +639,7 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
     S->k = k; S->len = st->len; S->n = n; S->base = st->base; S->slots = st->slots; S->wrap = wrap;
     S->threads = plan_threads(n);
     const uint64_t resident = lds_bytes_resident(n, k, S->threads);
-    S->global = C->opts.force_global || resident > C->max_lds;
+    S->global = C->opts.force_global || resident > C->max_dyn_lds;
     S->lds = S->global ? lds_scratch_bytes(S->threads) : resident;
     // Device memory: x, N'-hat, N-hat or A|B, work vectors and limbs in global mode, exponents
     // of up to k words, per-slot words, tables (about 6n elements per field, twice), a margin.
