@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Headline benchmark: modular squarings per second at 100,000-bit moduli on one MI355X.
+
+Workload (the reference's score workload, montmul/src/score.rs:26-28, 45-61, 93-123): random
+odd 100,000-bit moduli from GMP's generator seeded with 2026 (top bit and bit 0 set),
+k = 4,096 limbs, L = 8,192, base 3, exponent = the top 2,000 bits of N - 1 in every slot.
+A bench "step" is one pass of the hot path over the batch: re-arm every slot (load 1, set the
+exponent) and run(2,000), i.e. slots x 2,000 modular squarings. Warm-up passes run(64).
+
+The slot count is the best of 256 x {1, 2, 4, 8, 16} (CUs x {1, 2, 4, 8, 16}, as the
+reference's candidates, main.rs:199-216) at 200-bit exponents, unless --slots is given. The
+headline is global mode (transform vectors in HBM, as the workload specifies); LDS mode
+(vectors resident in LDS) is measured and reported beside it. Slot 0, the last slot and two
+seeded random slots must equal 3^e mod N (GMP powm) or no number is reported
+(score.rs:105-115).
+
+Prints one JSON line (the last line of stdout). Progress goes to stderr.
+"""
+
+import argparse
+import json
+import os
+import random
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "squarings/s at 100k-bit, 1 MI355X"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+REF_5090 = 1_536_441.0     # reference, RTX 5090, 170 slots (README.md:14)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def arm(batch, exps):
+    for i, e in enumerate(exps):
+        batch.load(i, 1)
+        batch.set_exponent(i, e)
+
+
+def median(v):
+    return statistics.median(v)
+
+
+def mad(v):
+    m = median(v)
+    return median([abs(x - m) for x in v])
+
+
+def measure(mm, hip, moduli, exps, slots, ebits, repeats, warmup):
+    """Warm-up run(64) passes, then `repeats` timed re-armed run(ebits) passes."""
+    b = mm.ModBatch(hip, moduli[:slots], 3)
+    ex = exps[:slots]
+    for _ in range(warmup):
+        arm(b, ex)
+        b.run(64)
+    rates, secs, kms = [], [], []
+    for _ in range(repeats):
+        arm(b, ex)
+        mm.Hip.push(b)  # inputs resident in HBM before the clock starts
+        t0 = time.perf_counter()
+        b.run(ebits)
+        dt = time.perf_counter() - t0
+        ms, launches = mm.Hip.last_run_kernel_ms(b)
+        rates.append(slots * ebits / dt)
+        secs.append(dt)
+        kms.append(ms / max(1, launches))
+    return b, rates, secs, kms
+
+
+def check(b, exps, seed):
+    slots = b.slots()
+    rng = random.Random(seed + 1)
+    picked = [0, slots - 1] + [rng.randrange(slots) for _ in range(2)]
+    for i in picked:
+        if not b.check_pow(i, 3, exps[i]):
+            raise SystemExit(f"slot {i} disagrees with pow_mod; no score reported")
+    return picked
+
+
+def sweep(mm, hip, moduli, exps, candidates, sweep_ebits):
+    best, res = None, {}
+    for s in candidates:
+        b, rates, _, _ = measure(mm, hip, moduli, exps, s, sweep_ebits, 1, 1)
+        b.free()
+        res[s] = rates[0]
+        log(f"  sweep slots {s}: {rates[0]:.4g} sq/s")
+        if best is None or rates[0] > res[best]:
+            best = s
+    return best, res
+
+
+def cpu_baseline(bits, ebits, threads):
+    """The oracle's Gmp restatement (closed-form REDC on GMP integers, gmp.rs) on this host's
+    cores, threads x 1 slots, and its Cpu (NTT) restatement on a short sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import montmul_amd as mm
+    from oracle_backends import Cpu, Gmp
+
+    moduli, exps = mm.score_workload(bits, threads, ebits, 2026)
+    out = {}
+    b = mm.ModBatch(Gmp(threads=threads), moduli, 3)
+    arm(b, exps)
+    b.run(16)
+    arm(b, exps)
+    t0 = time.perf_counter()
+    b.run(ebits)
+    dt = time.perf_counter() - t0
+    out["gmp"] = threads * ebits / dt
+    steps = 64
+    c = mm.ModBatch(Cpu(wraparound=True, threads=threads), moduli, 3)
+    arm(c, exps)
+    t0 = time.perf_counter()
+    c.run(steps)
+    dt2 = time.perf_counter() - t0
+    out["cpu_ntt"] = threads * steps / dt2
+    out["gmp_sample"] = f"{threads} slots x {ebits} steps, {bits}-bit moduli, {dt:.2f} s wall"
+    out["cpu_sample"] = f"{threads} slots x {steps} steps, {bits}-bit moduli, {dt2:.2f} s wall"
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed passes (repeats of run(ebits))")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed run(64) passes")
+    ap.add_argument("--slots", type=int, default=0, help="slot count (0: best of 256 x {1,2,4,8,16})")
+    ap.add_argument("--bits", type=int, default=100_000)
+    ap.add_argument("--ebits", type=int, default=2000)
+    ap.add_argument("--sweep-ebits", type=int, default=200)
+    ap.add_argument("--modes", default="global,lds")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (slots would shard "
+              "across devices with no exchange per step). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import montmul_amd as mm
+
+    t_setup = time.perf_counter()
+    maxslots = args.slots or 256 * 16
+    moduli, exps = mm.score_workload(args.bits, maxslots, args.ebits, 2026)
+    log(f"workload: {maxslots} moduli of {args.bits} bits in {time.perf_counter() - t_setup:.1f} s")
+    modes = [m for m in args.modes.split(",") if m]
+    results = {}
+    for mode in modes:
+        hip = mm.Hip(force_global=(mode == "global"))
+        cus = hip.cu_count()
+        if args.slots:
+            best, sw = args.slots, {}
+        else:
+            cands = [cus * f for f in (1, 2, 4, 8, 16)]
+            log(f"mode {mode}: sweep {cands}")
+            best, sw = sweep(mm, hip, moduli, exps, cands, args.sweep_ebits)
+        log(f"mode {mode}: {best} slots, {args.warmup} warm-up + {args.steps} timed passes of run({args.ebits})")
+        b, rates, secs, kms = measure(mm, hip, moduli, exps, best, args.ebits, args.steps, args.warmup)
+        picked = check(b, exps, 2026)
+        plan = mm.Hip.plan_of(b)
+        L, k = b.params().len, b.params().k
+        results[mode] = dict(slots=best, rates=rates, secs=secs, kernel_ms=kms, plan=plan, L=L, k=k,
+                             sweep={str(s): round(r) for s, r in sw.items()}, checked=picked)
+        log(f"mode {mode}: median {median(rates):.4g} sq/s, kernel {statistics.mean(kms):.1f} ms/launch")
+        b.free()
+        hip.close()
+
+    head = results.get("global") or results[modes[0]]
+    slots, ebits, L = head["slots"], args.ebits, head["L"]
+    total = slots * ebits * len(head["secs"])
+    value = total / sum(head["secs"])
+    kms = statistics.mean(head["kernel_ms"])
+    # Algorithmic HBM bytes per slot-step in global mode: the per-slot constant spectra N'-hat
+    # and A|B (12 L bytes each, read once per step) plus x read and written (2 L each): 28 L.
+    alg_bytes = 28 * L * slots * ebits
+    achieved = alg_bytes / (kms * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "squarings/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * statistics.mean(head["secs"]), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(value / REF_5090, 4),
+        "vs_baseline_ref": "reference Gpu backend, RTX 5090, 170 slots, 1,536,441 squarings/s (README.md:14)",
+        "dtype": "u32",
+        "data": "synthetic (GMP generator seeded 2026, as the reference's score workload)",
+        "config": {"workload": "100,000-bit moduli, L=8192, k=4096, base 3, top 2,000 bits of N-1, "
+                               "global mode (vectors in HBM)",
+                   "bits": args.bits, "L": L, "k": head["k"], "slots": slots, "ebits": ebits,
+                   "mode": "global" if "global" in results else modes[0],
+                   "threads_per_wg": head["plan"]["threads"], "lds_bytes": head["plan"]["lds_bytes"]},
+        "median": round(median(head["rates"]), 1),
+        "mad": round(mad(head["rates"]), 1),
+        "fermat_tests_per_hour": round(value * 3600 / args.bits, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
+                     "kernel": "k_steps (global mode)", "kernel_ms_per_launch": round(kms, 3),
+                     "alg_bytes_per_slot_step": 28 * L},
+        "checked_slots": head["checked"],
+        "sweep": head["sweep"],
+    }
+    if "lds" in results and "global" in results:
+        r = results["lds"]
+        line["lds_mode"] = {"value": round(r["slots"] * ebits * len(r["secs"]) / sum(r["secs"]), 1),
+                            "median": round(median(r["rates"]), 1), "mad": round(mad(r["rates"]), 1),
+                            "slots": r["slots"], "kernel_ms_per_launch": round(statistics.mean(r["kernel_ms"]), 3),
+                            "lds_bytes": r["plan"]["lds_bytes"], "sweep": r["sweep"]}
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        cb = cpu_baseline(args.bits, ebits, threads)
+        line["cpu_baseline"] = {"value": round(cb["gmp"], 1), "unit": "squarings/s", "cores": threads,
+                                "kind": "port", "sample": "oracle Gmp restatement (gmp.rs closed-form REDC): " + cb["gmp_sample"],
+                                "cpu_ntt_value": round(cb["cpu_ntt"], 1),
+                                "cpu_ntt_sample": "oracle Cpu restatement (cpu.rs NTT, wraparound): " + cb["cpu_sample"]}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -184,6 +184,10 @@ int mm_modbatch_modulus(mm_modbatch *b, size_t slot, uint32_t *out);
 int mm_fermat(const mm_backend *be, size_t count, const uint32_t *limbs, const size_t *offsets,
               const int8_t *signs, uint32_t base, uint64_t chunk, uint8_t *prp);
 
+/* The score's check of one slot (montmul/src/score.rs:107-115): no exponent bits left and
+ * read(slot) == base^e mod N, e given as little-endian limbs. 1 if so, 0 if not, < 0 on error. */
+int mm_modbatch_check_pow(mm_modbatch *b, size_t slot, uint32_t base, const uint32_t *e, size_t ne);
+
 /* The score workload (montmul/src/score.rs:45-61): `slots` moduli of `bits` bits drawn from
  * GMP's default random state seeded with `seed`, top bit and bit 0 set, and per modulus the top
  * `ebits` bits of N - 1. Writes ceil(bits/32) limbs per modulus and ceil(ebits/32) per exponent. */

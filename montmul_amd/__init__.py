@@ -118,6 +118,7 @@ def _load_lib():
         "mm_fermat": (C.c_int, [C.POINTER(BackendTable), C.c_size_t, u32p, C.POINTER(C.c_size_t),
                                 C.POINTER(C.c_int8), C.c_uint32, C.c_uint64, C.POINTER(C.c_uint8)]),
         "mm_score_workload": (C.c_int, [C.c_uint32, C.c_size_t, C.c_uint32, C.c_uint64, u32p, u32p]),
+        "mm_modbatch_check_pow": (C.c_int, [vp, C.c_size_t, C.c_uint32, u32p, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -378,6 +379,14 @@ class ModBatch:
         if lib.mm_modbatch_read(self._h, i, buf) != 0:
             raise MontmulError(_host_error())
         return from_limbs(buf)
+
+    def check_pow(self, i, base, e):
+        """No bits left and read(i) == base^e mod N, checked with GMP (score.rs:107-115)."""
+        buf = to_limbs(int(e))
+        r = lib.mm_modbatch_check_pow(self._h, i, base, buf, len(buf))
+        if r < 0:
+            raise MontmulError(_host_error())
+        return bool(r)
 
     def is_one(self, i):
         r = lib.mm_modbatch_is_one(self._h, i)

@@ -310,6 +310,22 @@ int mm_fermat(const mm_backend* be, size_t count, const uint32_t* limbs, const s
     return 0;
 }
 
+// The score's check of one slot (score.rs:107-115): no exponent bits left and
+// read(slot) == base^e mod N, with e given as little-endian limbs. 1 if it holds, 0 if not,
+// negative on error.
+int mm_modbatch_check_pow(mm_modbatch* b, size_t i, uint32_t base, const uint32_t* e, size_t ne) {
+    if (check_slot(b, i)) return -1;
+    if (b->be.remaining(b->state, i) != 0) return 0;
+    std::vector<uint32_t> l(b->k);
+    if (mm_modbatch_read(b, i, l.data())) return -1;
+    Mpz got, want, ez, bz;
+    mpz_import(got.v, b->k, -1, 4, 0, 0, l.data());
+    mpz_import(ez.v, ne, -1, 4, 0, 0, e);
+    mpz_set_ui(bz.v, base);
+    mpz_powm(want.v, bz.v, ez.v, b->moduli[i].n.v);
+    return mpz_cmp(got.v, want.v) == 0;
+}
+
 // The score workload (score.rs:45-61): `slots` moduli of `bits` bits from GMP's default
 // random state seeded with `seed` (rug's RandState::new + seed + Integer::random_bits), top bit
 // and bit 0 set, and per modulus the top `ebits` bits of N - 1, i.e. (N - 1) >> (bits - ebits).
