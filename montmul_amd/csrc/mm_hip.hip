@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -21,6 +22,7 @@
 
 #include "../../include/montmul_hip.h"
 #include "mm_step.h"
+#include "mm_v2_step.h"
 
 namespace mm {
 
@@ -66,6 +68,49 @@ __global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_
         if (!BARE)
             bad |= redc<WRAP>(w, x, k, A.t, A.ht, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
                               S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
+    }
+    if (tid() == 0) {
+        S.pos[slot] = pos;
+        if (bad) S.err[slot] = 1;
+    }
+}
+
+// The same steps on register-resident transforms (wraparound path, n = 512 .. 4,096): the
+// vector lives in registers, exchanges go through LDS (or the slot's HBM work buffer in
+// global mode).
+template <bool GLOBAL, bool BARE>
+__global__ __launch_bounds__(512) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const Slots& S = A.s;
+    const uint32_t slot = first + blockIdx.x;
+    const uint32_t n = A.t.P.n, k = S.k;
+    v2::Xbuf X;
+    uint32_t* Tl;
+    uint64_t* sc;
+    if (GLOBAL) {
+        X.z2 = (v2::Z2*)(S.work2 + (size_t)slot * n);
+        X.z1 = (v2::Z1*)(S.work1 + (size_t)slot * n);
+        Tl = S.lim + (size_t)slot * 3 * k;
+        sc = (uint64_t*)lds;
+    } else {
+        X.z2 = (v2::Z2*)lds;
+        X.z1 = (v2::Z1*)(lds + 16 * (size_t)n);
+        Tl = (uint32_t*)(lds + 24 * (size_t)n);
+        sc = (uint64_t*)(lds + 24 * (size_t)n + 8 * (size_t)k);
+    }
+    uint32_t* x = S.x + (size_t)slot * k;
+    const uint32_t* ex = S.exp + (size_t)slot * S.ew;
+    const uint32_t nb = S.nbits[slot];
+    uint32_t pos = S.pos[slot];
+    uint32_t bad = 0;
+    for (uint32_t it = 0; it < bits && pos < nb; it++, pos++) {
+        __syncthreads();  // x written by every thread in the previous step
+        const uint32_t bit = (ex[pos >> 5] >> (31 - (pos & 31))) & 1;
+        const uint32_t c = bit ? S.base : 1;
+        bad |= v2::v2_square(A.t, X, Tl, sc, x, c, A.crt_inv);
+        if (!BARE)
+            bad |= v2::v2_redc(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
+                               S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
     }
     if (tid() == 0) {
         S.pos[slot] = pos;
@@ -289,6 +334,70 @@ template <class F> HostTab<F> make_tab(uint32_t n, bool weights, bool unpack) {
     return T;
 }
 
+// Pass plan of the register transforms at length n (n >= 64): one radix-2 or radix-4 pass
+// first when log2 n is not a multiple of 3, then radix-8 passes down to stride 1.
+inline v2::Plan make_plan(uint32_t n) {
+    v2::Plan P{};
+    P.n = n;
+    P.logn = ilog2(n);
+    P.threads = n / 8;
+    uint32_t b = P.logn % 3, s = n, np = 0;
+    if (b) {
+        P.radix[np] = 1u << b;
+        s /= P.radix[np];
+        P.lstride[np++] = ilog2(s);
+    }
+    while (s > 1) {
+        P.radix[np] = 8;
+        s /= 8;
+        P.lstride[np++] = ilog2(s);
+    }
+    P.npass = np;
+    uint32_t off = 0;
+    for (uint32_t p = 0; p < np; p++) {
+        P.tw_off[p] = off;
+        off += (1u << P.lstride[p]) * (P.radix[p] - 1);
+    }
+    return P;
+}
+
+inline uint32_t host_revr(uint32_t m, uint32_t r) {
+    return r == 8 ? ((m & 1) << 2) | (m & 2) | (m >> 2) : r == 4 ? ((m & 1) << 1) | (m >> 1) : m;
+}
+
+// Pass twiddles of the plan: for pass p (radix r, stride s), j < s and slot m = 1 .. r-1, the
+// element omega_(r s)^(j rev_r(m)) (inverse roots for the DIT table).
+template <class F> std::vector<F> make_v2_twiddles(const v2::Plan& P, bool inverse) {
+    const uint32_t E = Ord<F>::E;
+    const F omega = cpow(gen<F>(), (u128)1 << (E - P.logn));
+    const F base = inverse ? cinv(omega) : omega;
+    std::vector<F> out;
+    for (uint32_t p = 0; p < P.npass; p++) {
+        const uint32_t r = P.radix[p], s = 1u << P.lstride[p];
+        const F root = cpow(base, (u128)(P.n / (r * s)));
+        F rj = cone<F>();
+        for (uint32_t j = 0; j < s; j++) {
+            F pw[8];
+            pw[0] = cone<F>();
+            for (uint32_t q = 1; q < r; q++) pw[q] = cmul(pw[q - 1], rj);
+            for (uint32_t m = 1; m < r; m++) out.push_back(pw[host_revr(m, r)]);
+            rj = cmul(rj, root);
+        }
+    }
+    return out;
+}
+
+// omega^(n/4) = i and omega^(n/8) = 2^h (1 + i) (h = 15 in M31, 30 in M61), which the radix-8
+// butterflies assume.
+inline bool check_small_roots() {
+    F1 w8a = cpow(gen<F1>(), (u128)1 << (Ord<F1>::E - 3));
+    F2 w8b = cpow(gen<F2>(), (u128)1 << (Ord<F2>::E - 3));
+    F1 w4a = cmul(w8a, w8a);
+    F2 w4b = cmul(w8b, w8b);
+    return w8a.re == (1u << 15) && w8a.im == (1u << 15) && w8b.re == (1ull << 30) && w8b.im == (1ull << 30) &&
+           w4a.re == 0 && w4a.im == 1 && w4b.re == 0 && w4b.im == 1;
+}
+
 }  // namespace mm
 
 using namespace mm;
@@ -304,6 +413,7 @@ int hip_fail(const char* what, hipError_t e) { return fail(std::string(what) + "
 struct DevTables {
     uint32_t n = 0;
     Tab t{};
+    v2::VTab vt{};  // register-transform tables (n >= 64)
     HalfTab ht{};  // tables at length n (the half path of length 2n uses these)
     std::vector<void*> bufs;
     int device = 0;
@@ -319,7 +429,8 @@ struct Ctx {
     hipStream_t stream = nullptr;
     int cus = 0;
     size_t max_lds = 0;
-    size_t max_dyn_lds = 0;  // dynamic LDS every step kernel can take
+    size_t max_dyn_lds = 0;     // dynamic LDS every step kernel can take
+    size_t max_dyn_lds_v2 = 0;  // the same for the register-transform kernels
     std::string name;
     std::mutex mu;
     std::map<uint32_t, std::shared_ptr<DevTables>> tabs;
@@ -364,6 +475,15 @@ int get_tables(Ctx* C, uint32_t n, std::shared_ptr<DevTables>* out) {
         upload_vec(*D, h2.iw, &t.iw2) || upload_vec(*D, h1.dtw, &t.dtw1) || upload_vec(*D, h2.dtw, &t.dtw2) ||
         upload_vec(*D, h1.itw, &t.itw1) || upload_vec(*D, h2.itw, &t.itw2))
         return 1;
+    if (n >= 64) {
+        if (!check_small_roots()) return fail("roots of unity differ from the radix-8 butterflies' assumption");
+        v2::VTab& V = D->vt;
+        V.P = make_plan(n);
+        V.w1 = t.w1; V.w2 = t.w2; V.iw1 = t.iw1; V.iw2 = t.iw2;
+        if (upload_vec(*D, make_v2_twiddles<F1>(V.P, false), &V.tw1) || upload_vec(*D, make_v2_twiddles<F2>(V.P, false), &V.tw2) ||
+            upload_vec(*D, make_v2_twiddles<F1>(V.P, true), &V.itw1) || upload_vec(*D, make_v2_twiddles<F2>(V.P, true), &V.itw2))
+            return 1;
+    }
     D->ht.t = t;
     if (upload_vec(*D, h1.u, &D->ht.u1) || upload_vec(*D, h1.iu, &D->ht.iu1) || upload_vec(*D, h2.u, &D->ht.u2) ||
         upload_vec(*D, h2.iu, &D->ht.iu2))
@@ -376,7 +496,7 @@ int get_tables(Ctx* C, uint32_t n, std::shared_ptr<DevTables>* out) {
 struct State {
     Ctx* C = nullptr;
     uint32_t k = 0, len = 0, n = 0, base = 0, slots = 0;
-    bool global = false, wrap = true;
+    bool global = false, wrap = true, v2 = false;
     uint32_t threads = 0;
     size_t lds = 0;
     std::shared_ptr<DevTables> tab, half;
@@ -399,6 +519,14 @@ struct State {
         for (void* p : {(void*)d_x, (void*)d_exp, (void*)d_nbits, (void*)d_pos, (void*)d_err, (void*)d_lim,
                         (void*)d_nphat1, (void*)d_nhat1, (void*)d_work1, (void*)d_nphat2, (void*)d_nhat2, (void*)d_work2})
             if (p) hipFree(p);
+    }
+    v2::VArgs vargs() const {
+        v2::VArgs A{};
+        A.t = tab->vt;
+        A.ht = half->vt;
+        A.s = args().s;
+        A.crt_inv = tab->t.crt_inv;
+        return A;
     }
     KArgs args() const {
         KArgs A{};
@@ -428,6 +556,11 @@ template <class T> int dalloc(T** p, size_t count) {
 }
 
 typedef void (*StepKernel)(KArgs, uint32_t, uint32_t);
+typedef void (*StepKernelV2)(v2::VArgs, uint32_t, uint32_t);
+StepKernelV2 pick_step_v2(bool global, bool bare) {
+    if (global) return bare ? k_steps_v2<true, true> : k_steps_v2<true, false>;
+    return bare ? k_steps_v2<false, true> : k_steps_v2<false, false>;
+}
 StepKernel pick_step(bool global, bool wrap, bool bare) {
     if (global) {
         if (bare) return wrap ? k_steps<true, true, true> : k_steps<true, false, true>;
@@ -453,6 +586,18 @@ int set_lds_limit(Ctx* C) {
             dyn = std::min<size_t>(dyn, 65536);
         }
         dyn_max = std::min(dyn_max, dyn);
+    }
+    const StepKernelV2 k2[] = {k_steps_v2<true, true>, k_steps_v2<true, false>, k_steps_v2<false, true>,
+                               k_steps_v2<false, false>};
+    for (StepKernelV2 k : k2) {
+        hipFuncAttributes a;
+        HIPCHK(hipFuncGetAttributes(&a, (const void*)k));
+        size_t dyn = C->max_lds > a.sharedSizeBytes ? C->max_lds - a.sharedSizeBytes : 0;
+        if (hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) != hipSuccess) {
+            (void)hipGetLastError();
+            dyn = std::min<size_t>(dyn, 65536);
+        }
+        C->max_dyn_lds_v2 = std::min(C->max_dyn_lds_v2 ? C->max_dyn_lds_v2 : dyn, dyn);
     }
     C->max_dyn_lds = dyn_max;
     return 0;
@@ -525,6 +670,12 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
     if (push(S)) return MM_EFAULT;
     KArgs A = S->args();
     StepKernel kern = pick_step(S->global, S->wrap, bare);
+    v2::VArgs A2{};
+    StepKernelV2 kern2 = nullptr;
+    if (S->v2) {
+        A2 = S->vargs();
+        kern2 = pick_step_v2(S->global, bare);
+    }
     while (left > 0) {
         // Steps per launch: all of them, or about max_launch_ms worth from the latest rate.
         uint64_t s = left;
@@ -535,7 +686,10 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
         }
         s = std::min<uint64_t>(s, 0xffffffffu);
         hipEventRecord(C->ev0, C->stream);
-        hipLaunchKernelGGL(kern, dim3(S->slots), dim3(S->threads), S->lds, C->stream, A, 0u, (uint32_t)s);
+        if (kern2)
+            hipLaunchKernelGGL(kern2, dim3(S->slots), dim3(S->threads), S->lds, C->stream, A2, 0u, (uint32_t)s);
+        else
+            hipLaunchKernelGGL(kern, dim3(S->slots), dim3(S->threads), S->lds, C->stream, A, 0u, (uint32_t)s);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) { hip_fail("step kernel launch", le); return MM_EFAULT; }
         hipEventRecord(C->ev1, C->stream);
@@ -637,10 +791,19 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
     std::unique_ptr<State> S(new State());
     S->C = C;
     S->k = k; S->len = st->len; S->n = n; S->base = st->base; S->slots = st->slots; S->wrap = wrap;
-    S->threads = plan_threads(n);
-    const uint64_t resident = lds_bytes_resident(n, k, S->threads);
-    S->global = C->opts.force_global || resident > C->max_dyn_lds;
-    S->lds = S->global ? lds_scratch_bytes(S->threads) : resident;
+    const char* kenv = getenv("MONTMUL_KERNEL");
+    S->v2 = wrap && n >= 512 && n <= 4096 && !(kenv && strcmp(kenv, "v1") == 0);
+    if (S->v2) {
+        S->threads = n / 8;
+        const uint64_t resident = v2::v2_lds_resident(n, k, S->threads);
+        S->global = C->opts.force_global || resident > C->max_dyn_lds_v2;
+        S->lds = S->global ? lds_scratch_bytes(S->threads) : resident;
+    } else {
+        S->threads = plan_threads(n);
+        const uint64_t resident = lds_bytes_resident(n, k, S->threads);
+        S->global = C->opts.force_global || resident > C->max_dyn_lds;
+        S->lds = S->global ? lds_scratch_bytes(S->threads) : resident;
+    }
     // Device memory: x, N'-hat, N-hat or A|B, work vectors and limbs in global mode, exponents
     // of up to k words, per-slot words, tables (about 6n elements per field, twice), a margin.
     const uint64_t per_slot = 4ull * k + 48ull * n + (S->global ? 24ull * n + 12ull * k : 0) + 16 + 4ull * k;

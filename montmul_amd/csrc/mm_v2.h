@@ -1,0 +1,420 @@
+// mm_v2.h - register-resident transforms for the fused step (lengths n = 512 .. 8,192).
+//
+// Each thread holds 8 elements of both fields in registers; a transform is a sequence of
+// radix-8 (or one radix-2/4) passes, each a butterfly over the thread's elements followed (DIF)
+// or preceded (DIT) by that pass's twiddles, with an exchange through a swizzled buffer
+// (LDS, or the slot's HBM work buffer in global mode) between passes. The pass layouts make
+// the pointwise steps exchange-free: after the last DIF pass thread t holds positions
+// 8t .. 8t+7, which is also where the first DIT pass reads.
+//
+// Arithmetic is lazy. GF(M61) values are 64-bit words <= 2^61 + 6 (a fold (x & P2) + (x >> 61)
+// after every operation keeps them there); GF(M31) values are 32-bit words in [0, P1], P1
+// standing for 0. Products go through four v_mad_u64_u32 into 128 bits and one Mersenne
+// fold; on gfx950 these multiplies issue at the rate of an add, so instruction count is the
+// cost. Constants read from memory (twiddles, weights, spectra) are canonical.
+//
+// The roots are those of the reference (field.rs:237-267): in both fields omega^(n/4) = i and
+// omega^(n/8) = 2^h (1 + i) with h = 30 (M61) and 15 (M31), so the radix-8 butterflies need
+// only rotations and adds internally. The host asserts both identities when it builds tables.
+#pragma once
+
+#include "mm_kernels.h"
+
+namespace mm {
+namespace v2 {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+// ---- Lazy GF(M61) ---------------------------------------------------------------------------
+constexpr u64 K4 = 4 * P2;  // a multiple of P2 above every lazy value
+
+__device__ __forceinline__ u64 f2(u64 x) { return (x & P2) + (x >> 61); }  // <= 2^61 + 6
+__device__ __forceinline__ u64 ad2(u64 a, u64 b) { return f2(a + b); }
+__device__ __forceinline__ u64 sb2(u64 a, u64 b) { return f2(a + (K4 - b)); }
+// x * 2^30 for x < 2^62: x = xh 2^31 + xl, 2^61 = 1, so x 2^30 = xh + xl 2^30.
+__device__ __forceinline__ u64 rot2(u64 x) {
+    u64 xl = (u32)x & 0x7fffffffu;
+    return (xl << 30) + (x >> 31);
+}
+// Canonical value of a lazy one.
+__device__ __forceinline__ u64 canon2(u64 x) { return x >= P2 ? x - P2 : x; }
+
+struct W128 {
+    u64 lo, hi;
+};
+// x * y for x, y < 2^62 as 128 bits: four 32x32->64 multiply-adds.
+__device__ __forceinline__ W128 mul64(u64 x, u64 y) {
+    u32 x0 = (u32)x, x1 = (u32)(x >> 32), y0 = (u32)y, y1 = (u32)(y >> 32);
+    u64 t = (u64)x0 * y0;
+    u64 u = (u64)x0 * y1 + (t >> 32);
+    u64 v = (u64)x1 * y0 + u;
+    u64 w = (u64)x1 * y1 + (v >> 32);
+    return W128{(v << 32) | (u32)t, w};
+}
+// P mod P2 (lazy) for P < 2^124.5: (P & P2) + (P >> 61), then a fold.
+__device__ __forceinline__ u64 red2w(u64 lo, u64 hi) { return f2((lo & P2) + ((hi << 3) | (lo >> 61))); }
+__device__ __forceinline__ u64 red2p(W128 p) { return red2w(p.lo, p.hi); }
+// a + b for two products each below 2^123.01 (one factor canonical).
+__device__ __forceinline__ u64 red2sum(W128 a, W128 b) {
+    u128 s = (((u128)a.hi << 64) | a.lo) + (((u128)b.hi << 64) | b.lo);
+    return red2w((u64)s, (u64)(s >> 64));
+}
+// x * y for a 32-bit x and a canonical y.
+__device__ __forceinline__ u64 mul32x2(u32 x, u64 y) {
+    u64 t = (u64)x * (u32)y;
+    u64 u = (u64)x * (u32)(y >> 32) + (t >> 32);  // P = u 2^32 + t_lo < 2^93
+    u64 lo = (u << 32) | (u32)t;
+    return f2((lo & P2) + (u >> 29));
+}
+
+// ---- Lazy GF(M31): words in [0, P1] -------------------------------------------------------
+__device__ __forceinline__ u32 ad1(u32 a, u32 b) { u32 s = a + b; return min(s, s - P1); }
+__device__ __forceinline__ u32 sb1(u32 a, u32 b) { u32 d = a - b; return min(d, d + P1); }
+__device__ __forceinline__ u32 rot1(u32 x) { return ((x << 15) & P1) | (x >> 16); }  // x * 2^15
+// P mod P1 into [0, P1] for P <= 2 P1^2.
+__device__ __forceinline__ u32 red1w(u64 p) {
+    u32 a = (u32)p & P1, b = (u32)(p >> 31);
+    u32 s = a + (b & P1) + (b >> 31);
+    return min(s, s - P1);
+}
+__device__ __forceinline__ u32 canon1(u32 x) { return x == P1 ? 0 : x; }
+
+// ---- Complex elements and per-field operations ----------------------------------------------
+struct Z1 {
+    u32 re, im;
+};
+struct alignas(16) Z2 {
+    u64 re, im;
+};
+
+struct Op1 {
+    typedef u32 B;
+    typedef Z1 Z;
+    typedef F1 C;  // canonical constant
+    static __device__ __forceinline__ B ad(B a, B b) { return ad1(a, b); }
+    static __device__ __forceinline__ B sb(B a, B b) { return sb1(a, b); }
+    static __device__ __forceinline__ B rot(B a) { return rot1(a); }
+    // x * w, w canonical.
+    static __device__ __forceinline__ Z mul(Z x, C w) {
+        u64 re = (u64)x.re * w.re + (u64)x.im * (P1 - w.im);
+        u64 im = (u64)x.re * w.im + (u64)x.im * w.re;
+        return Z{red1w(re), red1w(im)};
+    }
+    static __device__ __forceinline__ Z sqr(Z x) {
+        u64 re = (u64)x.re * x.re + (u64)x.im * (P1 - x.im);
+        u64 im = 2 * ((u64)x.re * x.im);
+        return Z{red1w(re), red1w(im)};
+    }
+    static __device__ __forceinline__ Z scale(Z x, u32 c) { return Z{red1w((u64)x.re * c), red1w((u64)x.im * c)}; }
+    // A real 32-bit value times a canonical weight.
+    static __device__ __forceinline__ Z real_mul(u32 v, C w) {
+        u32 m = mod1(v);
+        return Z{red1w((u64)m * w.re), red1w((u64)m * w.im)};
+    }
+    static __device__ __forceinline__ Z conj(Z x) { return Z{x.re, sb1(0, x.im)}; }
+    static __device__ __forceinline__ C canon(Z x) { return C{canon1(x.re), canon1(x.im)}; }
+};
+
+struct Op2 {
+    typedef u64 B;
+    typedef Z2 Z;
+    typedef F2 C;
+    static __device__ __forceinline__ B ad(B a, B b) { return ad2(a, b); }
+    static __device__ __forceinline__ B sb(B a, B b) { return sb2(a, b); }
+    static __device__ __forceinline__ B rot(B a) { return rot2(a); }
+    static __device__ __forceinline__ Z mul(Z x, C w) {
+        return Z{red2sum(mul64(x.re, w.re), mul64(x.im, P2 - w.im)), red2sum(mul64(x.re, w.im), mul64(x.im, w.re))};
+    }
+    static __device__ __forceinline__ Z sqr(Z x) {
+        u64 s = ad2(x.re, x.im), d = sb2(x.re, x.im);
+        u64 m = red2p(mul64(x.re, x.im));
+        return Z{red2p(mul64(s, d)), f2(m + m)};
+    }
+    static __device__ __forceinline__ Z scale(Z x, u32 c) {
+        // c <= 4: x c < 2^63.1.
+        return Z{f2((u64)(u32)x.re * c + (((u64)(u32)(x.re >> 32) * c) << 32)),
+                 f2((u64)(u32)x.im * c + (((u64)(u32)(x.im >> 32) * c) << 32))};
+    }
+    static __device__ __forceinline__ Z real_mul(u32 v, C w) { return Z{mul32x2(v, w.re), mul32x2(v, w.im)}; }
+    static __device__ __forceinline__ Z conj(Z x) { return Z{x.re, sb2(0, x.im)}; }
+    static __device__ __forceinline__ C canon(Z x) { return C{canon2(x.re), canon2(x.im)}; }
+};
+
+template <class O> __device__ __forceinline__ typename O::Z zadd(typename O::Z a, typename O::Z b) {
+    return {O::ad(a.re, b.re), O::ad(a.im, b.im)};
+}
+template <class O> __device__ __forceinline__ typename O::Z zsub(typename O::Z a, typename O::Z b) {
+    return {O::sb(a.re, b.re), O::sb(a.im, b.im)};
+}
+// (a - b) * i
+template <class O> __device__ __forceinline__ typename O::Z zsub_i(typename O::Z a, typename O::Z b) {
+    return {O::sb(b.im, a.im), O::sb(a.re, b.re)};
+}
+// (a - b) * (-i)
+template <class O> __device__ __forceinline__ typename O::Z zsub_mi(typename O::Z a, typename O::Z b) {
+    return {O::sb(a.im, b.im), O::sb(b.re, a.re)};
+}
+
+// ---- Butterflies over a thread's slots ------------------------------------------------------
+// DIF radix 8 on v[o], v[o+1], ..., v[o+7]: output slot m holds frequency rev3(m).
+template <class O> __device__ __forceinline__ void dif8(typename O::Z* v) {
+    typedef typename O::Z Z;
+    Z a0 = zadd<O>(v[0], v[4]), a1 = zadd<O>(v[1], v[5]), a2 = zadd<O>(v[2], v[6]), a3 = zadd<O>(v[3], v[7]);
+    Z a4 = zsub<O>(v[0], v[4]);
+    Z d1 = zsub<O>(v[1], v[5]);
+    Z a5{O::rot(O::sb(d1.re, d1.im)), O::rot(O::ad(d1.re, d1.im))};  // d1 2^h (1 + i)
+    Z a6 = zsub_i<O>(v[2], v[6]);                                    // (v2 - v6) i
+    Z e3 = zsub<O>(v[7], v[3]);                                      // -(v3 - v7)
+    Z a7{O::rot(O::ad(e3.re, e3.im)), O::rot(O::sb(e3.im, e3.re))};  // (v3 - v7) 2^h (-1 + i)
+    Z b0 = zadd<O>(a0, a2), b2 = zsub<O>(a0, a2), b1 = zadd<O>(a1, a3), b3 = zsub_i<O>(a1, a3);
+    Z b4 = zadd<O>(a4, a6), b6 = zsub<O>(a4, a6), b5 = zadd<O>(a5, a7), b7 = zsub_i<O>(a5, a7);
+    v[0] = zadd<O>(b0, b1);
+    v[1] = zsub<O>(b0, b1);
+    v[2] = zadd<O>(b2, b3);
+    v[3] = zsub<O>(b2, b3);
+    v[4] = zadd<O>(b4, b5);
+    v[5] = zsub<O>(b4, b5);
+    v[6] = zadd<O>(b6, b7);
+    v[7] = zsub<O>(b6, b7);
+}
+template <class O> __device__ __forceinline__ void dif4(typename O::Z* v) {
+    typedef typename O::Z Z;
+    Z a0 = zadd<O>(v[0], v[2]), a2 = zsub<O>(v[0], v[2]), a1 = zadd<O>(v[1], v[3]), a3 = zsub_i<O>(v[1], v[3]);
+    v[0] = zadd<O>(a0, a1);
+    v[1] = zsub<O>(a0, a1);
+    v[2] = zadd<O>(a2, a3);
+    v[3] = zsub<O>(a2, a3);
+}
+template <class O> __device__ __forceinline__ void dif2(typename O::Z* v) {
+    typename O::Z a = v[0], b = v[1];
+    v[0] = zadd<O>(a, b);
+    v[1] = zsub<O>(a, b);
+}
+// DIT radix 8 (inverse roots): input slot m holds frequency rev3(m), output natural.
+template <class O> __device__ __forceinline__ void dit8(typename O::Z* v) {
+    typedef typename O::Z Z;
+    Z b0 = zadd<O>(v[0], v[1]), b1 = zsub<O>(v[0], v[1]), b2 = zadd<O>(v[2], v[3]), b3 = zsub<O>(v[2], v[3]);
+    Z b4 = zadd<O>(v[4], v[5]), b5 = zsub<O>(v[4], v[5]), b6 = zadd<O>(v[6], v[7]), b7 = zsub<O>(v[6], v[7]);
+    // span 2: second element of pairs (1,3), (5,7) times -i.
+    Z c0 = zadd<O>(b0, b2), c2 = zsub<O>(b0, b2);
+    Z c1{O::ad(b1.re, b3.im), O::sb(b1.im, b3.re)}, c3{O::sb(b1.re, b3.im), O::ad(b1.im, b3.re)};
+    Z c4 = zadd<O>(b4, b6), c6 = zsub<O>(b4, b6);
+    Z c5{O::ad(b5.re, b7.im), O::sb(b5.im, b7.re)}, c7{O::sb(b5.re, b7.im), O::ad(b5.im, b7.re)};
+    // span 4: t_i = omega8^-i c_{i+4}: 1, 2^h (1 - i), -i, 2^h (-1 - i).
+    v[0] = zadd<O>(c0, c4);
+    v[4] = zsub<O>(c0, c4);
+    Z t1{O::rot(O::ad(c5.re, c5.im)), O::rot(O::sb(c5.im, c5.re))};
+    v[1] = zadd<O>(c1, t1);
+    v[5] = zsub<O>(c1, t1);
+    v[2] = Z{O::ad(c2.re, c6.im), O::sb(c2.im, c6.re)};
+    v[6] = Z{O::sb(c2.re, c6.im), O::ad(c2.im, c6.re)};
+    typename O::B u3 = O::rot(O::sb(c7.im, c7.re)), w3 = O::rot(O::ad(c7.re, c7.im));  // t3 = (u3, -w3)
+    v[3] = Z{O::ad(c3.re, u3), O::sb(c3.im, w3)};
+    v[7] = Z{O::sb(c3.re, u3), O::ad(c3.im, w3)};
+}
+template <class O> __device__ __forceinline__ void dit4(typename O::Z* v) {
+    typedef typename O::Z Z;
+    Z b0 = zadd<O>(v[0], v[1]), b1 = zsub<O>(v[0], v[1]), b2 = zadd<O>(v[2], v[3]), b3 = zsub<O>(v[2], v[3]);
+    v[0] = zadd<O>(b0, b2);
+    v[2] = zsub<O>(b0, b2);
+    v[1] = Z{O::ad(b1.re, b3.im), O::sb(b1.im, b3.re)};
+    v[3] = Z{O::sb(b1.re, b3.im), O::ad(b1.im, b3.re)};
+}
+template <class O> __device__ __forceinline__ void dit2(typename O::Z* v) { dif2<O>(v); }
+
+// ---- Pass plan ---------------------------------------------------------------------------
+// DIF passes in order: radix r_p (2, 4 or 8; only the first may be below 8) and stride s_p,
+// with s_0 = n / r_0 and s_{p+1} = s_p / r_{p+1}; the last stride is 1. tw_off[p] indexes the
+// pass's twiddles (s_p * (r_p - 1) entries: for j < s_p and slot m = 1 .. r_p - 1, the
+// twiddle omega_(r s)^(j rev_r(m)); the DIT table holds the inverses).
+constexpr int MAXPASS = 6;
+struct Plan {
+    u32 n, logn, threads, npass;
+    u32 radix[MAXPASS], lstride[MAXPASS], tw_off[MAXPASS];
+};
+
+// Radix-r bit reversal of m.
+__device__ __forceinline__ u32 revr(u32 m, u32 r) {
+    return r == 8 ? ((m & 1) << 2) | (m & 2) | (m >> 2) : r == 4 ? ((m & 1) << 1) | (m >> 1) : m;
+}
+
+// Element position of thread t's slot e in pass p (r = radix, ls = log2 stride, T threads).
+__device__ __forceinline__ u32 slot_pos(u32 t, u32 e, u32 r, u32 ls, u32 T) {
+    u32 u = e / r, m = e % r;
+    u32 g = u * T + t;
+    u32 b = g >> ls, j = g & ((1u << ls) - 1);
+    return b * r * (1u << ls) + j + (m << ls);
+}
+// Swizzled index in the exchange buffer (bank-conflict-free 8-lane groups).
+__device__ __forceinline__ u32 sw(u32 p) { return p ^ ((p >> 3) & 7); }
+
+// The exchange buffer: n elements of each field (and 3 words per coefficient when reused
+// for the carry's transposition).
+struct Xbuf {
+    Z2* z2;
+    Z1* z1;
+};
+
+// Butterfly over the thread's slots for a pass of radix r.
+template <class O, bool DIF> __device__ __forceinline__ void bfly(typename O::Z* v, u32 r) {
+    if (r == 8) {
+        if (DIF) dif8<O>(v); else dit8<O>(v);
+    } else if (r == 4) {
+        if (DIF) { dif4<O>(v); dif4<O>(v + 4); } else { dit4<O>(v); dit4<O>(v + 4); }
+    } else {
+#pragma unroll
+        for (int g = 0; g < 8; g += 2) dif2<O>(v + g);
+    }
+}
+
+// Twiddles of pass p for thread t, slots e with m != 0 (DIF: after, DIT: before the butterfly).
+template <class O>
+__device__ __forceinline__ void twiddle(typename O::Z* v, const typename O::C* tw, u32 t, u32 r, u32 ls, u32 T) {
+    if (ls == 0) return;  // stride 1: all twiddles are 1
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        u32 m = e % r;
+        if (m == 0) continue;
+        u32 g = (e / r) * T + t, j = g & ((1u << ls) - 1);
+        v[e] = O::mul(v[e], tw[j * (r - 1) + (m - 1)]);
+    }
+}
+
+// Writes the thread's slots of pass p to the exchange buffer at their positions.
+__device__ __forceinline__ void put(const Xbuf& X, const Z1* v1, const Z2* v2, u32 t, u32 r, u32 ls, u32 T) {
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        u32 q = sw(slot_pos(t, e, r, ls, T));
+        X.z2[q] = v2[e];
+        X.z1[q] = v1[e];
+    }
+}
+__device__ __forceinline__ void get(const Xbuf& X, Z1* v1, Z2* v2, u32 t, u32 r, u32 ls, u32 T) {
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        u32 q = sw(slot_pos(t, e, r, ls, T));
+        v2[e] = X.z2[q];
+        v1[e] = X.z1[q];
+    }
+}
+
+// Full DIF on registers laid out for pass 0; ends laid out for the last pass (positions
+// 8t .. 8t+7). Threads t >= P.threads hold no elements but join every barrier.
+__device__ void dif_regs(Z1* v1, Z2* v2, const Plan& P, const F1* tw1, const F2* tw2, const Xbuf& X) {
+    const u32 t = threadIdx.x, T = P.threads;
+    const bool act = t < T;
+    for (u32 p = 0; p < P.npass; p++) {
+        const u32 r = P.radix[p], ls = P.lstride[p];
+        if (act) {
+            bfly<Op1, true>(v1, r);
+            bfly<Op2, true>(v2, r);
+            twiddle<Op1>(v1, tw1 + P.tw_off[p], t, r, ls, T);
+            twiddle<Op2>(v2, tw2 + P.tw_off[p], t, r, ls, T);
+        }
+        if (p + 1 < P.npass) {
+            __syncthreads();
+            if (act) put(X, v1, v2, t, r, ls, T);
+            __syncthreads();
+            if (act) get(X, v1, v2, t, P.radix[p + 1], P.lstride[p + 1], T);
+        }
+    }
+}
+
+// Full DIT from the last pass's layout back to pass 0's (natural positions).
+__device__ void dit_regs(Z1* v1, Z2* v2, const Plan& P, const F1* itw1, const F2* itw2, const Xbuf& X) {
+    const u32 t = threadIdx.x, T = P.threads;
+    const bool act = t < T;
+    for (int p = (int)P.npass - 1; p >= 0; p--) {
+        const u32 r = P.radix[p], ls = P.lstride[p];
+        if (act) {
+            twiddle<Op1>(v1, itw1 + P.tw_off[p], t, r, ls, T);
+            twiddle<Op2>(v2, itw2 + P.tw_off[p], t, r, ls, T);
+            bfly<Op1, false>(v1, r);
+            bfly<Op2, false>(v2, r);
+        }
+        if (p > 0) {
+            __syncthreads();
+            if (act) put(X, v1, v2, t, r, ls, T);
+            __syncthreads();
+            if (act) get(X, v1, v2, t, P.radix[p - 1], P.lstride[p - 1], T);
+        }
+    }
+}
+
+// ---- Carry over registers ------------------------------------------------------------------
+// Thread t holds coefficients c[0..E) of positions E t .. E t + E - 1 (all blockDim threads,
+// count = blockDim * E). Writes the limbs of sum c_q 2^(32q) + cin to l and returns the carry
+// out of the top. Carry lookahead as in block_carry: neighbour high parts by shuffle/LDS, then
+// 0/1 carries by generate/propagate ballots over lanes and waves. sc: 2 * waves + 2 words.
+template <int E>
+__device__ u64 carry_regs(const u128* c, u32* l, u64* sc, u64 cin) {
+    const u32 t = threadIdx.x, nt = blockDim.x, lane = t & 63, wave = t >> 6, nw = nt >> 6;
+    u128 acc = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        acc += c[e];
+        l[e] = (u32)acc;
+        acc >>= 32;
+    }
+    const u64 hi = (u64)acc;
+    const u64 up = __shfl_up(hi, 1, 64);
+    __syncthreads();
+    if (lane == 63) sc[wave] = hi;
+    __syncthreads();
+    u64 k = t == 0 ? cin : (lane == 0 ? sc[wave - 1] : up);
+    bool ones = true;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        u64 v = (u64)l[e] + k;
+        l[e] = (u32)v;
+        k = v >> 32;
+        ones &= l[e] == 0xffffffffu;
+    }
+    const u32 owner = nt - 1;
+    const bool live = t < owner;
+    const u64 g = __ballot(live && k != 0);
+    const u64 y = g | __ballot(live && ones);
+    const u64 s0 = g + y;
+    const bool wgen = s0 < g, wprop = y == ~0ull;
+    if (lane == 0) sc[nw + wave] = (u64)wgen | ((u64)wprop << 1);
+    __syncthreads();
+    const u64 f = lane < nw ? sc[nw + lane] : 0;
+    const u64 gw = __ballot(f & 1), yw = gw | __ballot((f & 2) != 0);
+    const u64 cw = ((gw + yw) ^ gw ^ yw) >> wave & 1;
+    const u64 s = g + y + cw;
+    u64 bit = ((s ^ g ^ y) >> lane) & 1;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        u64 v = (u64)l[e] + bit;
+        l[e] = (u32)v;
+        bit = v >> 32;
+    }
+    if (t == owner) sc[2 * nw] = hi + k + bit;
+    __syncthreads();
+    return sc[2 * nw];
+}
+
+// Coefficient staging for the transposition: 3 words per coefficient in the exchange
+// buffer's space: two words in the M61 half (16n bytes) and one in the M31 half (8n bytes),
+// 2n coefficients in all.
+struct Cbuf {
+    u32 *w0, *w1, *w2;
+};
+__device__ __forceinline__ Cbuf cbuf(const Xbuf& X, u32 n) {
+    u32* b2 = (u32*)X.z2;
+    return Cbuf{b2, b2 + 2 * n, (u32*)X.z1};
+}
+__device__ __forceinline__ void cput(const Cbuf& C, u32 q, u128 v) {
+    C.w0[q] = (u32)v;
+    C.w1[q] = (u32)(v >> 32);
+    C.w2[q] = (u32)(v >> 64);
+}
+__device__ __forceinline__ u128 cget(const Cbuf& C, u32 q) {
+    return (u128)C.w0[q] | ((u128)C.w1[q] << 32) | ((u128)C.w2[q] << 64);
+}
+
+__device__ __forceinline__ u128 crt_lazy(u32 c1, u64 c2, u32 inv) { return crt(canon1(c1), canon2(c2), inv); }
+
+}  // namespace v2
+}  // namespace mm

@@ -1,0 +1,183 @@
+// mm_v2_step.h - the fused Montgomery step on register-resident transforms (wraparound path).
+#pragma once
+
+#include "mm_v2.h"
+
+namespace mm {
+namespace v2 {
+
+// Device tables of one length for the register transforms.
+struct VTab {
+    Plan P;
+    const F1 *w1, *iw1;    // zeta^j, zeta^-j / n (canonical)
+    const F2 *w2, *iw2;
+    const F1 *tw1, *itw1;  // pass twiddles (layout of Plan)
+    const F2 *tw2, *itw2;
+};
+
+struct VArgs {
+    VTab t;     // length n
+    VTab ht;    // length h = n / 2 (weights unused)
+    Slots s;
+    u32 crt_inv;
+};
+
+// LDS bytes of the LDS-resident layout: exchange buffer (24 n), T (8 k), carry scratch.
+__host__ __device__ inline uint64_t v2_lds_resident(uint32_t n, uint32_t k, uint32_t threads) {
+    return 24ull * n + 8ull * k + 8ull * (2 * (threads / 64) + 2);
+}
+
+// One step's squaring: T = c x^2 into Tl (2k limbs); returns nonzero on a carry out.
+__device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, u64* sc, const u32* x, u32 c, u32 inv) {
+    const u32 t = threadIdx.x, T = V.P.threads, n = V.P.n;
+    const u32 r0 = V.P.radix[0], ls0 = V.P.lstride[0];
+    Z1 v1[8];
+    Z2 v2[8];
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        u32 pos = slot_pos(t, e, r0, ls0, T);
+        u32 xv = x[pos];
+        v1[e] = Op1::real_mul(xv, V.w1[pos]);
+        v2[e] = Op2::real_mul(xv, V.w2[pos]);
+    }
+    dif_regs(v1, v2, V.P, V.tw1, V.tw2, X);
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        v1[e] = Op1::sqr(v1[e]);
+        v2[e] = Op2::sqr(v2[e]);
+        if (c != 1) {
+            v1[e] = Op1::scale(v1[e], c);
+            v2[e] = Op2::scale(v2[e], c);
+        }
+    }
+    dit_regs(v1, v2, V.P, V.itw1, V.itw2, X);
+    const Cbuf C = cbuf(X, n);
+    __syncthreads();
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        u32 pos = slot_pos(t, e, r0, ls0, T);
+        Z1 a = Op1::mul(v1[e], V.iw1[pos]);
+        Z2 b = Op2::mul(v2[e], V.iw2[pos]);
+        cput(C, pos, crt_lazy(a.re, b.re, inv));
+        cput(C, pos + n, crt_lazy(a.im, b.im, inv));
+    }
+    __syncthreads();
+    u128 cq[16];
+#pragma unroll
+    for (u32 i = 0; i < 16; i++) cq[i] = cget(C, 16 * t + i);
+    u32 l[16];
+    u64 top = carry_regs<16>(cq, l, sc, 0);
+#pragma unroll
+    for (u32 i = 0; i < 16; i++) Tl[16 * t + i] = l[i];
+    return top != 0;
+}
+
+// Reduction: x = T R^-1 (lazy) from T in Tl, through m = (T mod R) N' mod R and the cyclic
+// product m N mod (R - 1) (montmul/src/cpu.rs:452-484, 297-328). Returns nonzero on a carry
+// that must vanish but does not.
+__device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u64* sc, u32* x, const F1* np1, const F2* np2,
+                                       const F1* ab1, const F2* ab2) {
+    const VTab& V = A.t;
+    const VTab& H = A.ht;
+    const u32 t = threadIdx.x, T = V.P.threads, n = V.P.n, k = n, h = n / 2, Th = H.P.threads;
+    const u32 r0 = V.P.radix[0], ls0 = V.P.lstride[0];
+    const u32 inv = A.crt_inv;
+    const Cbuf C = cbuf(X, n);
+    Z1 v1[8];
+    Z2 v2[8];
+    __syncthreads();  // T's limbs written by every thread
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        u32 pos = slot_pos(t, e, r0, ls0, T);
+        u32 tv = Tl[pos];
+        v1[e] = Op1::real_mul(tv, V.w1[pos]);
+        v2[e] = Op2::real_mul(tv, V.w2[pos]);
+    }
+    dif_regs(v1, v2, V.P, V.tw1, V.tw2, X);
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        v1[e] = Op1::mul(v1[e], np1[8 * t + e]);
+        v2[e] = Op2::mul(v2[e], np2[8 * t + e]);
+    }
+    dit_regs(v1, v2, V.P, V.itw1, V.itw2, X);
+    __syncthreads();
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        u32 pos = slot_pos(t, e, r0, ls0, T);
+        Z1 a = Op1::mul(v1[e], V.iw1[pos]);
+        Z2 b = Op2::mul(v2[e], V.iw2[pos]);
+        cput(C, pos, crt_lazy(a.re, b.re, inv));
+    }
+    __syncthreads();
+    u128 cm[8];
+#pragma unroll
+    for (u32 i = 0; i < 8; i++) cm[i] = cget(C, 8 * t + i);
+    u32 m[8];
+    carry_regs<8>(cm, m, sc, 0);
+    // m packed as z_j = m_2j + i m_2j+1 at half-length positions j = 4t .. 4t+3.
+#pragma unroll
+    for (u32 i = 0; i < 4; i++) {
+        u32 q = sw(4 * t + i);
+        X.z1[q] = Z1{mod1(m[2 * i]), mod1(m[2 * i + 1])};
+        X.z2[q] = Z2{m[2 * i], m[2 * i + 1]};
+    }
+    __syncthreads();
+    const bool hact = t < Th;
+    if (hact) get(X, v1, v2, t, H.P.radix[0], H.P.lstride[0], Th);
+    dif_regs(v1, v2, H.P, H.tw1, H.tw2, X);
+    // Pointwise with the partner position: out_p = (z_p + conj z_q) A_p + (z_p - conj z_q) B_p.
+    __syncthreads();
+    if (hact) {
+#pragma unroll
+        for (u32 e = 0; e < 8; e++) {
+            u32 q = sw(8 * t + e);
+            X.z1[q] = v1[e];
+            X.z2[q] = v2[e];
+        }
+    }
+    __syncthreads();
+    if (hact) {
+#pragma unroll
+        for (u32 e = 0; e < 8; e++) {
+            u32 p = 8 * t + e, q = sw(partner(p));
+            Z1 zq1 = Op1::conj(X.z1[q]);
+            Z2 zq2 = Op2::conj(X.z2[q]);
+            Z1 e1 = zadd<Op1>(v1[e], zq1), d1 = zsub<Op1>(v1[e], zq1);
+            Z2 e2 = zadd<Op2>(v2[e], zq2), d2 = zsub<Op2>(v2[e], zq2);
+            v1[e] = zadd<Op1>(Op1::mul(e1, ab1[p]), Op1::mul(d1, ab1[h + p]));
+            v2[e] = zadd<Op2>(Op2::mul(e2, ab2[p]), Op2::mul(d2, ab2[h + p]));
+        }
+    }
+    dit_regs(v1, v2, H.P, H.itw1, H.itw2, X);
+    // Cyclic product element j carries coefficients 2j (re) and 2j + 1 (im).
+    __syncthreads();
+    if (hact) {
+#pragma unroll
+        for (u32 e = 0; e < 8; e++) {
+            u32 j = slot_pos(t, e, H.P.radix[0], H.P.lstride[0], Th);
+            cput(C, 2 * j, crt_lazy(v1[e].re, v2[e].re, inv));
+            cput(C, 2 * j + 1, crt_lazy(v1[e].im, v2[e].im, inv));
+        }
+    }
+    __syncthreads();
+    // G = (cyclic + T mod R) mod (R - 1), x = floor(T / R) + G.
+    u128 cy[8];
+#pragma unroll
+    for (u32 i = 0; i < 8; i++) cy[i] = cget(C, 8 * t + i) + Tl[8 * t + i];
+    u32 xs[8];
+    u64 cc = carry_regs<8>(cy, xs, sc, 0);
+    for (int guard = 0; cc != 0 && guard < 4; guard++) {
+#pragma unroll
+        for (u32 i = 0; i < 8; i++) cy[i] = xs[i];
+        cc = carry_regs<8>(cy, xs, sc, cc);
+    }
+#pragma unroll
+    for (u32 i = 0; i < 8; i++) cy[i] = (u128)xs[i] + Tl[k + 8 * t + i];
+    u64 top = carry_regs<8>(cy, xs, sc, 0);
+#pragma unroll
+    for (u32 i = 0; i < 8; i++) x[8 * t + i] = xs[i];
+    return (top != 0) | (cc != 0);
+}
+
+}  // namespace v2
+}  // namespace mm
