@@ -78,7 +78,7 @@ __global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_
 // The same steps on register-resident transforms (wraparound path, n = 512 .. 4,096): the
 // vector lives in registers, exchanges go through LDS (or the slot's HBM work buffer in
 // global mode).
-template <bool GLOBAL, bool BARE>
+template <uint32_t LOGN, bool GLOBAL, bool BARE>
 __global__ __launch_bounds__(512) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
     extern __shared__ __align__(16) unsigned char lds[];
     const Slots& S = A.s;
@@ -107,9 +107,9 @@ __global__ __launch_bounds__(512) void k_steps_v2(v2::VArgs A, uint32_t first, u
         __syncthreads();  // x written by every thread in the previous step
         const uint32_t bit = (ex[pos >> 5] >> (31 - (pos & 31))) & 1;
         const uint32_t c = bit ? S.base : 1;
-        bad |= v2::v2_square(A.t, X, Tl, sc, x, c, A.crt_inv);
+        bad |= v2::v2_square<LOGN>(A.t, X, Tl, sc, x, c, A.crt_inv);
         if (!BARE)
-            bad |= v2::v2_redc(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
+            bad |= v2::v2_redc<LOGN>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
                                S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
     }
     if (tid() == 0) {
@@ -557,9 +557,17 @@ template <class T> int dalloc(T** p, size_t count) {
 
 typedef void (*StepKernel)(KArgs, uint32_t, uint32_t);
 typedef void (*StepKernelV2)(v2::VArgs, uint32_t, uint32_t);
-StepKernelV2 pick_step_v2(bool global, bool bare) {
-    if (global) return bare ? k_steps_v2<true, true> : k_steps_v2<true, false>;
-    return bare ? k_steps_v2<false, true> : k_steps_v2<false, false>;
+template <uint32_t LOGN> StepKernelV2 pick_v2(bool global, bool bare) {
+    if (global) return bare ? k_steps_v2<LOGN, true, true> : k_steps_v2<LOGN, true, false>;
+    return bare ? k_steps_v2<LOGN, false, true> : k_steps_v2<LOGN, false, false>;
+}
+StepKernelV2 pick_step_v2(uint32_t logn, bool global, bool bare) {
+    switch (logn) {
+        case 9: return pick_v2<9>(global, bare);
+        case 10: return pick_v2<10>(global, bare);
+        case 11: return pick_v2<11>(global, bare);
+        default: return pick_v2<12>(global, bare);
+    }
 }
 StepKernel pick_step(bool global, bool wrap, bool bare) {
     if (global) {
@@ -587,9 +595,8 @@ int set_lds_limit(Ctx* C) {
         }
         dyn_max = std::min(dyn_max, dyn);
     }
-    const StepKernelV2 k2[] = {k_steps_v2<true, true>, k_steps_v2<true, false>, k_steps_v2<false, true>,
-                               k_steps_v2<false, false>};
-    for (StepKernelV2 k : k2) {
+    for (uint32_t i = 0; i < 16; i++) {
+        StepKernelV2 k = pick_step_v2(9 + i / 4, (i >> 1) & 1, i & 1);
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, (const void*)k));
         size_t dyn = C->max_lds > a.sharedSizeBytes ? C->max_lds - a.sharedSizeBytes : 0;
@@ -674,7 +681,7 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
     StepKernelV2 kern2 = nullptr;
     if (S->v2) {
         A2 = S->vargs();
-        kern2 = pick_step_v2(S->global, bare);
+        kern2 = pick_step_v2(ilog2(S->n), S->global, bare);
     }
     while (left > 0) {
         // Steps per launch: all of them, or about max_launch_ms worth from the latest rate.

@@ -299,45 +299,77 @@ __device__ __forceinline__ void get(const Xbuf& X, Z1* v1, Z2* v2, u32 t, u32 r,
     }
 }
 
+// The thread index through an opaque move: address arithmetic derived from it cannot be
+// hoisted out of the step loop (where, for every slot and table, it would exhaust registers).
+__device__ __forceinline__ u32 opaque_tid() {
+    u32 t;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"(threadIdx.x));
+    return t;
+}
+
+// Scheduling fence: keeps the compiler from interleaving phases (and hoisting their loads),
+// which would multiply live registers across the unrolled transform.
+#define MM_FENCE() __builtin_amdgcn_sched_barrier(0)
+
+// The pass plan of length 2^LOGN at compile time (the same plan as make_plan on the host).
+template <u32 LOGN> struct CPlan {
+    static constexpr u32 n = 1u << LOGN, T = n / 8, b = LOGN % 3;
+    static constexpr u32 npass = (b ? 1 : 0) + LOGN / 3;
+    static constexpr u32 radix(u32 p) { return (b && p == 0) ? (1u << b) : 8; }
+    static constexpr u32 lstride(u32 p) { return p == 0 ? LOGN - (b ? b : 3) : lstride(p - 1) - 3; }
+    static constexpr u32 tw_off(u32 p) { return p == 0 ? 0 : tw_off(p - 1) + (1u << lstride(p - 1)) * (radix(p - 1) - 1); }
+};
+
 // Full DIF on registers laid out for pass 0; ends laid out for the last pass (positions
-// 8t .. 8t+7). Threads t >= P.threads hold no elements but join every barrier.
-__device__ void dif_regs(Z1* v1, Z2* v2, const Plan& P, const F1* tw1, const F2* tw2, const Xbuf& X) {
-    const u32 t = threadIdx.x, T = P.threads;
-    const bool act = t < T;
-    for (u32 p = 0; p < P.npass; p++) {
-        const u32 r = P.radix[p], ls = P.lstride[p];
+// 8t .. 8t+7). Threads t >= n / 8 hold no elements but join every barrier.
+template <u32 LOGN>
+__device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, const F2* tw2, const Xbuf& X) {
+    typedef CPlan<LOGN> P;
+    const bool act = t < P::T;
+#pragma unroll
+    for (u32 p = 0; p < P::npass; p++) {
+        const u32 r = P::radix(p), ls = P::lstride(p);
         if (act) {
             bfly<Op1, true>(v1, r);
+            twiddle<Op1>(v1, tw1 + P::tw_off(p), t, r, ls, P::T);
+            MM_FENCE();
             bfly<Op2, true>(v2, r);
-            twiddle<Op1>(v1, tw1 + P.tw_off[p], t, r, ls, T);
-            twiddle<Op2>(v2, tw2 + P.tw_off[p], t, r, ls, T);
+            MM_FENCE();
+            twiddle<Op2>(v2, tw2 + P::tw_off(p), t, r, ls, P::T);
+            MM_FENCE();
         }
-        if (p + 1 < P.npass) {
+        if (p + 1 < P::npass) {
             __syncthreads();
-            if (act) put(X, v1, v2, t, r, ls, T);
+            if (act) put(X, v1, v2, t, r, ls, P::T);
             __syncthreads();
-            if (act) get(X, v1, v2, t, P.radix[p + 1], P.lstride[p + 1], T);
+            if (act) get(X, v1, v2, t, P::radix(p + 1), P::lstride(p + 1), P::T);
         }
     }
 }
 
 // Full DIT from the last pass's layout back to pass 0's (natural positions).
-__device__ void dit_regs(Z1* v1, Z2* v2, const Plan& P, const F1* itw1, const F2* itw2, const Xbuf& X) {
-    const u32 t = threadIdx.x, T = P.threads;
-    const bool act = t < T;
-    for (int p = (int)P.npass - 1; p >= 0; p--) {
-        const u32 r = P.radix[p], ls = P.lstride[p];
+template <u32 LOGN>
+__device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, const F2* itw2, const Xbuf& X) {
+    typedef CPlan<LOGN> P;
+    const bool act = t < P::T;
+#pragma unroll
+    for (u32 q = 0; q < P::npass; q++) {
+        const u32 p = P::npass - 1 - q;
+        const u32 r = P::radix(p), ls = P::lstride(p);
         if (act) {
-            twiddle<Op1>(v1, itw1 + P.tw_off[p], t, r, ls, T);
-            twiddle<Op2>(v2, itw2 + P.tw_off[p], t, r, ls, T);
+            twiddle<Op1>(v1, itw1 + P::tw_off(p), t, r, ls, P::T);
             bfly<Op1, false>(v1, r);
+            MM_FENCE();
+            twiddle<Op2>(v2, itw2 + P::tw_off(p), t, r, ls, P::T);
+            MM_FENCE();
             bfly<Op2, false>(v2, r);
+            MM_FENCE();
         }
         if (p > 0) {
             __syncthreads();
-            if (act) put(X, v1, v2, t, r, ls, T);
+            if (act) put(X, v1, v2, t, r, ls, P::T);
             __syncthreads();
-            if (act) get(X, v1, v2, t, P.radix[p - 1], P.lstride[p - 1], T);
+            if (act) get(X, v1, v2, t, P::radix(p - 1), P::lstride(p - 1), P::T);
         }
     }
 }
@@ -347,13 +379,13 @@ __device__ void dit_regs(Z1* v1, Z2* v2, const Plan& P, const F1* itw1, const F2
 // count = blockDim * E). Writes the limbs of sum c_q 2^(32q) + cin to l and returns the carry
 // out of the top. Carry lookahead as in block_carry: neighbour high parts by shuffle/LDS, then
 // 0/1 carries by generate/propagate ballots over lanes and waves. sc: 2 * waves + 2 words.
-template <int E>
-__device__ u64 carry_regs(const u128* c, u32* l, u64* sc, u64 cin) {
-    const u32 t = threadIdx.x, nt = blockDim.x, lane = t & 63, wave = t >> 6, nw = nt >> 6;
+template <int E, class Coef>
+__device__ __forceinline__ u64 carry_regs(u32 t, Coef coef, u32* l, u64* sc, u64 cin) {
+    const u32 nt = blockDim.x, lane = t & 63, wave = t >> 6, nw = nt >> 6;
     u128 acc = 0;
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        acc += c[e];
+        acc += coef(e);
         l[e] = (u32)acc;
         acc >>= 32;
     }

@@ -28,9 +28,11 @@ __host__ __device__ inline uint64_t v2_lds_resident(uint32_t n, uint32_t k, uint
 }
 
 // One step's squaring: T = c x^2 into Tl (2k limbs); returns nonzero on a carry out.
+template <u32 LOGN>
 __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, u64* sc, const u32* x, u32 c, u32 inv) {
-    const u32 t = threadIdx.x, T = V.P.threads, n = V.P.n;
-    const u32 r0 = V.P.radix[0], ls0 = V.P.lstride[0];
+    typedef CPlan<LOGN> P;
+    const u32 t = opaque_tid(), T = P::T, n = P::n;
+    const u32 r0 = P::radix(0), ls0 = P::lstride(0);
     Z1 v1[8];
     Z2 v2[8];
 #pragma unroll
@@ -40,7 +42,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         v1[e] = Op1::real_mul(xv, V.w1[pos]);
         v2[e] = Op2::real_mul(xv, V.w2[pos]);
     }
-    dif_regs(v1, v2, V.P, V.tw1, V.tw2, X);
+    dif_regs<LOGN>(t, v1, v2, V.tw1, V.tw2, X);
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
         v1[e] = Op1::sqr(v1[e]);
@@ -50,7 +52,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
             v2[e] = Op2::scale(v2[e], c);
         }
     }
-    dit_regs(v1, v2, V.P, V.itw1, V.itw2, X);
+    dit_regs<LOGN>(t, v1, v2, V.itw1, V.itw2, X);
     const Cbuf C = cbuf(X, n);
     __syncthreads();
 #pragma unroll
@@ -62,11 +64,8 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         cput(C, pos + n, crt_lazy(a.im, b.im, inv));
     }
     __syncthreads();
-    u128 cq[16];
-#pragma unroll
-    for (u32 i = 0; i < 16; i++) cq[i] = cget(C, 16 * t + i);
     u32 l[16];
-    u64 top = carry_regs<16>(cq, l, sc, 0);
+    u64 top = carry_regs<16>(t, [&](int i) { return cget(C, 16 * t + i); }, l, sc, 0);
 #pragma unroll
     for (u32 i = 0; i < 16; i++) Tl[16 * t + i] = l[i];
     return top != 0;
@@ -75,12 +74,15 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
 // Reduction: x = T R^-1 (lazy) from T in Tl, through m = (T mod R) N' mod R and the cyclic
 // product m N mod (R - 1) (montmul/src/cpu.rs:452-484, 297-328). Returns nonzero on a carry
 // that must vanish but does not.
+template <u32 LOGN>
 __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u64* sc, u32* x, const F1* np1, const F2* np2,
                                        const F1* ab1, const F2* ab2) {
     const VTab& V = A.t;
     const VTab& H = A.ht;
-    const u32 t = threadIdx.x, T = V.P.threads, n = V.P.n, k = n, h = n / 2, Th = H.P.threads;
-    const u32 r0 = V.P.radix[0], ls0 = V.P.lstride[0];
+    typedef CPlan<LOGN> P;
+    typedef CPlan<LOGN - 1> PH;
+    const u32 t = opaque_tid(), T = P::T, n = P::n, k = n, h = n / 2, Th = PH::T;
+    const u32 r0 = P::radix(0), ls0 = P::lstride(0);
     const u32 inv = A.crt_inv;
     const Cbuf C = cbuf(X, n);
     Z1 v1[8];
@@ -93,13 +95,13 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         v1[e] = Op1::real_mul(tv, V.w1[pos]);
         v2[e] = Op2::real_mul(tv, V.w2[pos]);
     }
-    dif_regs(v1, v2, V.P, V.tw1, V.tw2, X);
+    dif_regs<LOGN>(t, v1, v2, V.tw1, V.tw2, X);
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
         v1[e] = Op1::mul(v1[e], np1[8 * t + e]);
         v2[e] = Op2::mul(v2[e], np2[8 * t + e]);
     }
-    dit_regs(v1, v2, V.P, V.itw1, V.itw2, X);
+    dit_regs<LOGN>(t, v1, v2, V.itw1, V.itw2, X);
     __syncthreads();
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
@@ -109,11 +111,8 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         cput(C, pos, crt_lazy(a.re, b.re, inv));
     }
     __syncthreads();
-    u128 cm[8];
-#pragma unroll
-    for (u32 i = 0; i < 8; i++) cm[i] = cget(C, 8 * t + i);
     u32 m[8];
-    carry_regs<8>(cm, m, sc, 0);
+    carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i); }, m, sc, 0);
     // m packed as z_j = m_2j + i m_2j+1 at half-length positions j = 4t .. 4t+3.
 #pragma unroll
     for (u32 i = 0; i < 4; i++) {
@@ -123,8 +122,8 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     }
     __syncthreads();
     const bool hact = t < Th;
-    if (hact) get(X, v1, v2, t, H.P.radix[0], H.P.lstride[0], Th);
-    dif_regs(v1, v2, H.P, H.tw1, H.tw2, X);
+    if (hact) get(X, v1, v2, t, PH::radix(0), PH::lstride(0), Th);
+    dif_regs<LOGN - 1>(t, v1, v2, H.tw1, H.tw2, X);
     // Pointwise with the partner position: out_p = (z_p + conj z_q) A_p + (z_p - conj z_q) B_p.
     __syncthreads();
     if (hact) {
@@ -148,32 +147,31 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
             v2[e] = zadd<Op2>(Op2::mul(e2, ab2[p]), Op2::mul(d2, ab2[h + p]));
         }
     }
-    dit_regs(v1, v2, H.P, H.itw1, H.itw2, X);
+    dit_regs<LOGN - 1>(t, v1, v2, H.itw1, H.itw2, X);
     // Cyclic product element j carries coefficients 2j (re) and 2j + 1 (im).
     __syncthreads();
     if (hact) {
 #pragma unroll
         for (u32 e = 0; e < 8; e++) {
-            u32 j = slot_pos(t, e, H.P.radix[0], H.P.lstride[0], Th);
+            u32 j = slot_pos(t, e, PH::radix(0), PH::lstride(0), Th);
             cput(C, 2 * j, crt_lazy(v1[e].re, v2[e].re, inv));
             cput(C, 2 * j + 1, crt_lazy(v1[e].im, v2[e].im, inv));
         }
     }
     __syncthreads();
     // G = (cyclic + T mod R) mod (R - 1), x = floor(T / R) + G.
-    u128 cy[8];
-#pragma unroll
-    for (u32 i = 0; i < 8; i++) cy[i] = cget(C, 8 * t + i) + Tl[8 * t + i];
     u32 xs[8];
-    u64 cc = carry_regs<8>(cy, xs, sc, 0);
+    u64 cc = carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i) + Tl[8 * t + i]; }, xs, sc, 0);
     for (int guard = 0; cc != 0 && guard < 4; guard++) {
+        u32 prev[8];
 #pragma unroll
-        for (u32 i = 0; i < 8; i++) cy[i] = xs[i];
-        cc = carry_regs<8>(cy, xs, sc, cc);
+        for (u32 i = 0; i < 8; i++) prev[i] = xs[i];
+        cc = carry_regs<8>(t, [&](int i) { return (u128)prev[i]; }, xs, sc, cc);
     }
+    u32 lo[8];
 #pragma unroll
-    for (u32 i = 0; i < 8; i++) cy[i] = (u128)xs[i] + Tl[k + 8 * t + i];
-    u64 top = carry_regs<8>(cy, xs, sc, 0);
+    for (u32 i = 0; i < 8; i++) lo[i] = xs[i];
+    u64 top = carry_regs<8>(t, [&](int i) { return (u128)lo[i] + Tl[k + 8 * t + i]; }, xs, sc, 0);
 #pragma unroll
     for (u32 i = 0; i < 8; i++) x[8 * t + i] = xs[i];
     return (top != 0) | (cc != 0);
