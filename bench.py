@@ -193,7 +193,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic (GMP generator seeded 2026, as the reference's score workload)",
         "config": {"workload": "100,000-bit moduli, L=8192, k=4096, base 3, top 2,000 bits of N-1, "
-                               "global mode (vectors in HBM)",
+                               "global mode (T, x and constants in HBM; transform exchanges through LDS tiles of at most 2,048 elements)",
                    "bits": args.bits, "L": L, "k": head["k"], "slots": slots, "ebits": ebits,
                    "mode": "global" if "global" in results else modes[0],
                    "threads_per_wg": head["plan"]["threads"], "lds_bytes": head["plan"]["lds_bytes"]},

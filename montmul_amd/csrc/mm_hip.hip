@@ -88,10 +88,13 @@ __global__ __launch_bounds__(512) void k_steps_v2(v2::VArgs A, uint32_t first, u
     uint32_t* Tl;
     uint64_t* sc;
     if (GLOBAL) {
-        X.z2 = (v2::Z2*)(S.work2 + (size_t)slot * n);
-        X.z1 = (v2::Z1*)(S.work1 + (size_t)slot * n);
+        // Transform vectors in registers between exchanges, exchanges through an LDS tile of
+        // at most 2,048 elements, T and x in HBM.
+        constexpr uint32_t tile = v2::Tiling<LOGN, true>::tile;
+        X.z2 = (v2::Z2*)lds;
+        X.z1 = (v2::Z1*)(lds + 16 * tile);
         Tl = S.lim + (size_t)slot * 3 * k;
-        sc = (uint64_t*)lds;
+        sc = (uint64_t*)(lds + 24 * tile);
     } else {
         X.z2 = (v2::Z2*)lds;
         X.z1 = (v2::Z1*)(lds + 16 * (size_t)n);
@@ -107,9 +110,9 @@ __global__ __launch_bounds__(512) void k_steps_v2(v2::VArgs A, uint32_t first, u
         __syncthreads();  // x written by every thread in the previous step
         const uint32_t bit = (ex[pos >> 5] >> (31 - (pos & 31))) & 1;
         const uint32_t c = bit ? S.base : 1;
-        bad |= v2::v2_square<LOGN>(A.t, X, Tl, sc, x, c, A.crt_inv);
+        bad |= v2::v2_square<LOGN, GLOBAL>(A.t, X, Tl, sc, x, c, A.crt_inv);
         if (!BARE)
-            bad |= v2::v2_redc<LOGN>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
+            bad |= v2::v2_redc<LOGN, GLOBAL>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
                                S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
     }
     if (tid() == 0) {
@@ -804,7 +807,8 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         S->threads = n / 8;
         const uint64_t resident = v2::v2_lds_resident(n, k, S->threads);
         S->global = C->opts.force_global || resident > C->max_dyn_lds_v2;
-        S->lds = S->global ? lds_scratch_bytes(S->threads) : resident;
+        const uint64_t tile = std::min<uint32_t>(n, 2048);
+        S->lds = S->global ? 24 * tile + lds_scratch_bytes(S->threads) : resident;
     } else {
         S->threads = plan_threads(n);
         const uint64_t resident = lds_bytes_resident(n, k, S->threads);

@@ -320,9 +320,70 @@ template <u32 LOGN> struct CPlan {
     static constexpr u32 tw_off(u32 p) { return p == 0 ? 0 : tw_off(p - 1) + (1u << lstride(p - 1)) * (radix(p - 1) - 1); }
 };
 
+// Largest exchange tile in elements: with TILED (global mode) the exchange buffer holds at
+// most 2,048 elements of both fields (48 KiB), and a longer vector is exchanged in chunks.
+constexpr u32 TILE_MAX = 2048;
+template <u32 LOGN, bool TILED> struct Tiling {
+    static constexpr u32 n = 1u << LOGN;
+    static constexpr u32 tile = (TILED && n > TILE_MAX) ? TILE_MAX : n;
+    static constexpr u32 chunks = n / tile;
+};
+
+// Moves the thread's slots from pass (ro, lso)'s layout to pass (ri, lsi)'s through the
+// exchange buffer, one tile-sized chunk of positions at a time. With several chunks the inputs
+// land in separate registers until every chunk's outputs are written (a thread's input slot
+// can lie in an earlier chunk than the output slot held in the same register). Starts and
+// ends with the barriers that order it against the buffer's other uses.
+template <u32 LOGN, bool TILED>
+__device__ __forceinline__ void exchange(u32 t, bool act, Z1* v1, Z2* v2, const Xbuf& X, u32 ro, u32 lso, u32 ri, u32 lsi) {
+    typedef Tiling<LOGN, TILED> G;
+    constexpr u32 T = CPlan<LOGN>::T;
+    if constexpr (G::chunks == 1) {
+        __syncthreads();
+        if (act) put(X, v1, v2, t, ro, lso, T);
+        __syncthreads();
+        if (act) get(X, v1, v2, t, ri, lsi, T);
+    } else {
+        Z1 n1[8];
+        Z2 n2[8];
+#pragma unroll
+        for (u32 c = 0; c < G::chunks; c++) {
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (u32 e = 0; e < 8; e++) {
+                    u32 pos = slot_pos(t, e, ro, lso, T);
+                    if (pos / G::tile == c) {
+                        u32 q = sw(pos % G::tile);
+                        X.z2[q] = v2[e];
+                        X.z1[q] = v1[e];
+                    }
+                }
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (u32 e = 0; e < 8; e++) {
+                    u32 pos = slot_pos(t, e, ri, lsi, T);
+                    if (pos / G::tile == c) {
+                        u32 q = sw(pos % G::tile);
+                        n2[e] = X.z2[q];
+                        n1[e] = X.z1[q];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (u32 e = 0; e < 8; e++) {
+            v1[e] = n1[e];
+            v2[e] = n2[e];
+        }
+    }
+}
+
 // Full DIF on registers laid out for pass 0; ends laid out for the last pass (positions
 // 8t .. 8t+7). Threads t >= n / 8 hold no elements but join every barrier.
-template <u32 LOGN>
+template <u32 LOGN, bool TILED>
 __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, const F2* tw2, const Xbuf& X) {
     typedef CPlan<LOGN> P;
     const bool act = t < P::T;
@@ -338,17 +399,12 @@ __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, c
             twiddle<Op2>(v2, tw2 + P::tw_off(p), t, r, ls, P::T);
             MM_FENCE();
         }
-        if (p + 1 < P::npass) {
-            __syncthreads();
-            if (act) put(X, v1, v2, t, r, ls, P::T);
-            __syncthreads();
-            if (act) get(X, v1, v2, t, P::radix(p + 1), P::lstride(p + 1), P::T);
-        }
+        if (p + 1 < P::npass) exchange<LOGN, TILED>(t, act, v1, v2, X, r, ls, P::radix(p + 1), P::lstride(p + 1));
     }
 }
 
 // Full DIT from the last pass's layout back to pass 0's (natural positions).
-template <u32 LOGN>
+template <u32 LOGN, bool TILED>
 __device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, const F2* itw2, const Xbuf& X) {
     typedef CPlan<LOGN> P;
     const bool act = t < P::T;
@@ -365,12 +421,7 @@ __device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, 
             bfly<Op2, false>(v2, r);
             MM_FENCE();
         }
-        if (p > 0) {
-            __syncthreads();
-            if (act) put(X, v1, v2, t, r, ls, P::T);
-            __syncthreads();
-            if (act) get(X, v1, v2, t, P::radix(p - 1), P::lstride(p - 1), P::T);
-        }
+        if (p > 0) exchange<LOGN, TILED>(t, act, v1, v2, X, r, ls, P::radix(p - 1), P::lstride(p - 1));
     }
 }
 

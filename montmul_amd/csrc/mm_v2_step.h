@@ -28,7 +28,7 @@ __host__ __device__ inline uint64_t v2_lds_resident(uint32_t n, uint32_t k, uint
 }
 
 // One step's squaring: T = c x^2 into Tl (2k limbs); returns nonzero on a carry out.
-template <u32 LOGN>
+template <u32 LOGN, bool TILED>
 __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, u64* sc, const u32* x, u32 c, u32 inv) {
     typedef CPlan<LOGN> P;
     const u32 t = opaque_tid(), T = P::T, n = P::n;
@@ -42,7 +42,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         v1[e] = Op1::real_mul(xv, V.w1[pos]);
         v2[e] = Op2::real_mul(xv, V.w2[pos]);
     }
-    dif_regs<LOGN>(t, v1, v2, V.tw1, V.tw2, X);
+    dif_regs<LOGN, TILED>(t, v1, v2, V.tw1, V.tw2, X);
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
         v1[e] = Op1::sqr(v1[e]);
@@ -52,20 +52,51 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
             v2[e] = Op2::scale(v2[e], c);
         }
     }
-    dit_regs<LOGN>(t, v1, v2, V.itw1, V.itw2, X);
-    const Cbuf C = cbuf(X, n);
-    __syncthreads();
-#pragma unroll
-    for (u32 e = 0; e < 8; e++) {
-        u32 pos = slot_pos(t, e, r0, ls0, T);
-        Z1 a = Op1::mul(v1[e], V.iw1[pos]);
-        Z2 b = Op2::mul(v2[e], V.iw2[pos]);
-        cput(C, pos, crt_lazy(a.re, b.re, inv));
-        cput(C, pos + n, crt_lazy(a.im, b.im, inv));
-    }
-    __syncthreads();
+    dit_regs<LOGN, TILED>(t, v1, v2, V.itw1, V.itw2, X);
+    typedef Tiling<LOGN, TILED> G;
     u32 l[16];
-    u64 top = carry_regs<16>(t, [&](int i) { return cget(C, 16 * t + i); }, l, sc, 0);
+    u64 top;
+    if constexpr (G::chunks == 1) {
+        const Cbuf C = cbuf(X, n);
+        __syncthreads();
+#pragma unroll
+        for (u32 e = 0; e < 8; e++) {
+            u32 pos = slot_pos(t, e, r0, ls0, T);
+            Z1 a = Op1::mul(v1[e], V.iw1[pos]);
+            Z2 b = Op2::mul(v2[e], V.iw2[pos]);
+            cput(C, pos, crt_lazy(a.re, b.re, inv));
+            cput(C, pos + n, crt_lazy(a.im, b.im, inv));
+        }
+        __syncthreads();
+        top = carry_regs<16>(t, [&](int i) { return cget(C, 16 * t + i); }, l, sc, 0);
+    } else {
+        // The tile holds 2 * tile coefficients: the real parts (coefficients 0 .. n-1) go
+        // through it first, then the imaginary parts (n .. 2n-1), each thread keeping its 16.
+        static_assert(2 * G::tile == n, "two coefficient chunks");
+        const Cbuf C = cbuf(X, G::tile);
+        u128 cre[8], cim[8];
+#pragma unroll
+        for (u32 e = 0; e < 8; e++) {
+            u32 pos = slot_pos(t, e, r0, ls0, T);
+            Z1 a = Op1::mul(v1[e], V.iw1[pos]);
+            Z2 b = Op2::mul(v2[e], V.iw2[pos]);
+            cre[e] = crt_lazy(a.re, b.re, inv);
+            cim[e] = crt_lazy(a.im, b.im, inv);
+        }
+        u128 mine[16];
+#pragma unroll
+        for (u32 c = 0; c < 2; c++) {
+            __syncthreads();
+#pragma unroll
+            for (u32 e = 0; e < 8; e++) cput(C, slot_pos(t, e, r0, ls0, T), c == 0 ? cre[e] : cim[e]);
+            __syncthreads();
+            if ((16 * t) / n == c) {
+#pragma unroll
+                for (u32 i = 0; i < 16; i++) mine[i] = cget(C, 16 * t + i - c * n);
+            }
+        }
+        top = carry_regs<16>(t, [&](int i) { return mine[i]; }, l, sc, 0);
+    }
 #pragma unroll
     for (u32 i = 0; i < 16; i++) Tl[16 * t + i] = l[i];
     return top != 0;
@@ -74,7 +105,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
 // Reduction: x = T R^-1 (lazy) from T in Tl, through m = (T mod R) N' mod R and the cyclic
 // product m N mod (R - 1) (montmul/src/cpu.rs:452-484, 297-328). Returns nonzero on a carry
 // that must vanish but does not.
-template <u32 LOGN>
+template <u32 LOGN, bool TILED>
 __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u64* sc, u32* x, const F1* np1, const F2* np2,
                                        const F1* ab1, const F2* ab2) {
     const VTab& V = A.t;
@@ -84,7 +115,9 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     const u32 t = opaque_tid(), T = P::T, n = P::n, k = n, h = n / 2, Th = PH::T;
     const u32 r0 = P::radix(0), ls0 = P::lstride(0);
     const u32 inv = A.crt_inv;
-    const Cbuf C = cbuf(X, n);
+    // Staging for k = n coefficients: the exchange buffer (one tile) holds 2 * tile of them.
+    static_assert(2 * Tiling<LOGN, TILED>::tile >= (1u << LOGN), "coefficients fit the tile");
+    const Cbuf C = cbuf(X, Tiling<LOGN, TILED>::tile);
     Z1 v1[8];
     Z2 v2[8];
     __syncthreads();  // T's limbs written by every thread
@@ -95,13 +128,13 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         v1[e] = Op1::real_mul(tv, V.w1[pos]);
         v2[e] = Op2::real_mul(tv, V.w2[pos]);
     }
-    dif_regs<LOGN>(t, v1, v2, V.tw1, V.tw2, X);
+    dif_regs<LOGN, TILED>(t, v1, v2, V.tw1, V.tw2, X);
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
         v1[e] = Op1::mul(v1[e], np1[8 * t + e]);
         v2[e] = Op2::mul(v2[e], np2[8 * t + e]);
     }
-    dit_regs<LOGN>(t, v1, v2, V.itw1, V.itw2, X);
+    dit_regs<LOGN, TILED>(t, v1, v2, V.itw1, V.itw2, X);
     __syncthreads();
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
@@ -123,7 +156,7 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     __syncthreads();
     const bool hact = t < Th;
     if (hact) get(X, v1, v2, t, PH::radix(0), PH::lstride(0), Th);
-    dif_regs<LOGN - 1>(t, v1, v2, H.tw1, H.tw2, X);
+    dif_regs<LOGN - 1, TILED>(t, v1, v2, H.tw1, H.tw2, X);
     // Pointwise with the partner position: out_p = (z_p + conj z_q) A_p + (z_p - conj z_q) B_p.
     __syncthreads();
     if (hact) {
@@ -147,7 +180,7 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
             v2[e] = zadd<Op2>(Op2::mul(e2, ab2[p]), Op2::mul(d2, ab2[h + p]));
         }
     }
-    dit_regs<LOGN - 1>(t, v1, v2, H.itw1, H.itw2, X);
+    dit_regs<LOGN - 1, TILED>(t, v1, v2, H.itw1, H.itw2, X);
     // Cyclic product element j carries coefficients 2j (re) and 2j + 1 (im).
     __syncthreads();
     if (hact) {
