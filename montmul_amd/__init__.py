@@ -21,7 +21,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmontmul_hip.so")
+LIB_PATH = os.environ.get("MONTMUL_LIB") or os.path.join(_HERE, "libmontmul_hip.so")
 
 __all__ = ["MontmulError", "CarryError", "Backend", "Hip", "ModBatch", "Params", "fermat",
            "score_workload", "lib", "LIB_PATH", "MM_OK", "MM_EFAULT"]

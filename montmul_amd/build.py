@@ -37,18 +37,18 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build(verbose=False):
+def build(verbose=False, out=LIB, extra=()):
     os.makedirs(OUT_DIR, exist_ok=True)
     hip_obj = os.path.join(OUT_DIR, "mm_hip.o")
     host_obj = os.path.join(OUT_DIR, "mm_host.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "mm_hip.hip"), "-o", hip_obj], verbose)
+    _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, "mm_hip.hip"), "-o", hip_obj], verbose)
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{GMP_INCLUDE}", "-c",
           os.path.join(CSRC, "mm_host.cpp"), "-o", host_obj], verbose)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, host_obj,
           GMP_LIB, "-pthread"], verbose)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

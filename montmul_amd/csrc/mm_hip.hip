@@ -79,7 +79,10 @@ __global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_
 // vector lives in registers, exchanges go through LDS (or the slot's HBM work buffer in
 // global mode).
 template <uint32_t LOGN, bool GLOBAL, bool BARE>
-__global__ __launch_bounds__(512) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
+#ifndef MM_V2_WAVES_PER_EU
+#define MM_V2_WAVES_PER_EU 2
+#endif
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_PER_EU))) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
     extern __shared__ __align__(16) unsigned char lds[];
     const Slots& S = A.s;
     const uint32_t slot = first + blockIdx.x;
