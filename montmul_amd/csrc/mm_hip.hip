@@ -76,12 +76,13 @@ __global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_
 }
 
 // The same steps on register-resident transforms (wraparound path, n = 512 .. 4,096): the
-// vector lives in registers, exchanges go through LDS (or the slot's HBM work buffer in
-// global mode).
-template <uint32_t LOGN, bool GLOBAL, bool BARE>
+// vector lives in registers, exchanges go through LDS; in global mode T and x stay in HBM and
+// the exchanges use an LDS tile of at most 2,048 elements. The kernels are held to 128 VGPRs
+// (four waves per SIMD), so two 512-thread workgroups share a CU wherever LDS allows.
 #ifndef MM_V2_WAVES_PER_EU
-#define MM_V2_WAVES_PER_EU 2
+#define MM_V2_WAVES_PER_EU 4
 #endif
+template <uint32_t LOGN, bool GLOBAL, bool BARE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_PER_EU))) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
     extern __shared__ __align__(16) unsigned char lds[];
     const Slots& S = A.s;
@@ -112,8 +113,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES
     for (uint32_t it = 0; it < bits && pos < nb; it++, pos++) {
         __syncthreads();  // x written by every thread in the previous step
         const uint32_t bit = (ex[pos >> 5] >> (31 - (pos & 31))) & 1;
-        const uint32_t c = bit ? S.base : 1;
-        bad |= v2::v2_square<LOGN, GLOBAL>(A.t, X, Tl, sc, x, c, A.crt_inv);
+        const F1* iw1 = bit ? A.iwc1 : A.t.iw1;
+        const v2::S2* iw2 = bit ? A.iwc2 : A.t.iw2;
+        bad |= v2::v2_square<LOGN, GLOBAL>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv);
         if (!BARE)
             bad |= v2::v2_redc<LOGN, GLOBAL>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
                                S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
@@ -122,6 +124,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES
         S.pos[slot] = pos;
         if (bad) S.err[slot] = 1;
     }
+}
+
+// The unweighting tables times c (a batch's base) for the register-transform step: a 1 bit's
+// factor c enters there instead of as a pointwise product (c IDFT(X) = IDFT(c X)).
+__global__ void k_scale_tab(const F1* iw1, const v2::S2* iw2, uint32_t c, uint32_t n, F1* o1, v2::S2* o2) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const F1 a = iw1[j];
+    o1[j] = F1{mul1(a.re, c), mul1(a.im, c)};
+    const v2::S2 b = iw2[j];
+    const uint64_t re = ((uint64_t)b.r1 << 31) | b.r0, im = ((uint64_t)b.i1 << 31) | b.i0;
+    o2[j] = v2::spl_c(F2{mul2(re, c), mul2(im, c)});
 }
 
 // Weighted forward transform of k-limb strings into out (n elements per vector), in place in
@@ -393,6 +407,13 @@ template <class F> std::vector<F> make_v2_twiddles(const v2::Plan& P, bool inver
     return out;
 }
 
+// GF(M61^2) constants in the split form of the register transforms' products.
+inline std::vector<v2::S2> split_all(const std::vector<F2>& v) {
+    std::vector<v2::S2> out(v.size());
+    for (size_t i = 0; i < v.size(); i++) out[i] = v2::spl_c(v[i]);
+    return out;
+}
+
 // omega^(n/4) = i and omega^(n/8) = 2^h (1 + i) (h = 15 in M31, 30 in M61), which the radix-8
 // butterflies assume.
 inline bool check_small_roots() {
@@ -485,9 +506,10 @@ int get_tables(Ctx* C, uint32_t n, std::shared_ptr<DevTables>* out) {
         if (!check_small_roots()) return fail("roots of unity differ from the radix-8 butterflies' assumption");
         v2::VTab& V = D->vt;
         V.P = make_plan(n);
-        V.w1 = t.w1; V.w2 = t.w2; V.iw1 = t.iw1; V.iw2 = t.iw2;
-        if (upload_vec(*D, make_v2_twiddles<F1>(V.P, false), &V.tw1) || upload_vec(*D, make_v2_twiddles<F2>(V.P, false), &V.tw2) ||
-            upload_vec(*D, make_v2_twiddles<F1>(V.P, true), &V.itw1) || upload_vec(*D, make_v2_twiddles<F2>(V.P, true), &V.itw2))
+        V.w1 = t.w1; V.iw1 = t.iw1;
+        if (upload_vec(*D, split_all(h2.w), &V.w2) || upload_vec(*D, split_all(h2.iw), &V.iw2) ||
+            upload_vec(*D, make_v2_twiddles<F1>(V.P, false), &V.tw1) || upload_vec(*D, split_all(make_v2_twiddles<F2>(V.P, false)), &V.tw2) ||
+            upload_vec(*D, make_v2_twiddles<F1>(V.P, true), &V.itw1) || upload_vec(*D, split_all(make_v2_twiddles<F2>(V.P, true)), &V.itw2))
             return 1;
     }
     D->ht.t = t;
@@ -510,6 +532,8 @@ struct State {
     uint32_t *d_x = nullptr, *d_exp = nullptr, *d_nbits = nullptr, *d_pos = nullptr, *d_err = nullptr, *d_lim = nullptr;
     F1 *d_nphat1 = nullptr, *d_nhat1 = nullptr, *d_work1 = nullptr;
     F2 *d_nphat2 = nullptr, *d_nhat2 = nullptr, *d_work2 = nullptr;
+    F1* d_iwc1 = nullptr;  // unweighting times base (register-transform step)
+    v2::S2* d_iwc2 = nullptr;
     uint32_t ew = 0;
     // host mirrors
     std::vector<uint32_t> h_x;
@@ -523,7 +547,8 @@ struct State {
     ~State() {
         hipSetDevice(C->device);
         for (void* p : {(void*)d_x, (void*)d_exp, (void*)d_nbits, (void*)d_pos, (void*)d_err, (void*)d_lim,
-                        (void*)d_nphat1, (void*)d_nhat1, (void*)d_work1, (void*)d_nphat2, (void*)d_nhat2, (void*)d_work2})
+                        (void*)d_nphat1, (void*)d_nhat1, (void*)d_work1, (void*)d_nphat2, (void*)d_nhat2, (void*)d_work2,
+                        (void*)d_iwc1, (void*)d_iwc2})
             if (p) hipFree(p);
     }
     v2::VArgs vargs() const {
@@ -532,6 +557,8 @@ struct State {
         A.ht = half->vt;
         A.s = args().s;
         A.crt_inv = tab->t.crt_inv;
+        A.iwc1 = d_iwc1;
+        A.iwc2 = d_iwc2;
         return A;
     }
     KArgs args() const {
@@ -834,6 +861,12 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         return 1;
     S->ew = 1;
     if (S->global && (dalloc(&S->d_work1, sl * n) || dalloc(&S->d_work2, sl * n) || dalloc(&S->d_lim, sl * 3 * k))) return 1;
+    if (S->v2) {
+        if (dalloc(&S->d_iwc1, n) || dalloc(&S->d_iwc2, n)) return 1;
+        hipLaunchKernelGGL(k_scale_tab, dim3((n + 255) / 256), dim3(256), 0, C->stream, S->tab->vt.iw1, S->tab->vt.iw2,
+                           st->base, n, S->d_iwc1, S->d_iwc2);
+        HIPCHK(hipGetLastError());
+    }
     hipStream_t sm = C->stream;
     HIPCHK(hipMemsetAsync(S->d_x, 0, sl * k * 4, sm));
     HIPCHK(hipMemsetAsync(S->d_err, 0, sl * 4, sm));

@@ -29,56 +29,74 @@ typedef uint32_t u32;
 // ---- Lazy GF(M61) ---------------------------------------------------------------------------
 constexpr u64 K4 = 4 * P2;  // a multiple of P2 above every lazy value
 
-__device__ __forceinline__ u64 f2(u64 x) { return (x & P2) + (x >> 61); }  // <= 2^61 + 6
-__device__ __forceinline__ u64 ad2(u64 a, u64 b) { return f2(a + b); }
-__device__ __forceinline__ u64 sb2(u64 a, u64 b) { return f2(a + (K4 - b)); }
+MM_HD u64 f2(u64 x) { return (x & P2) + (x >> 61); }  // < 2^64 in, <= 2^61 + 6 out
+MM_HD u64 ad2(u64 a, u64 b) { return f2(a + b); }
+MM_HD u64 sb2(u64 a, u64 b) { return f2(a + (K4 - b)); }
 // x * 2^30 for x < 2^62: x = xh 2^31 + xl, 2^61 = 1, so x 2^30 = xh + xl 2^30.
-__device__ __forceinline__ u64 rot2(u64 x) {
+MM_HD u64 rot2(u64 x) {
     u64 xl = (u32)x & 0x7fffffffu;
     return (xl << 30) + (x >> 31);
 }
 // Canonical value of a lazy one.
-__device__ __forceinline__ u64 canon2(u64 x) { return x >= P2 ? x - P2 : x; }
+MM_HD u64 canon2(u64 x) { return x >= P2 ? x - P2 : x; }
 
-struct W128 {
-    u64 lo, hi;
+// ---- GF(M61) products on 31-bit halves ------------------------------------------------------
+// A lazy word x <= 2^61 + 6 is x1 2^31 + x0 with x0 < 2^31 and x1 <= 2^30. With a second word
+// y = y1 2^31 + y0 (y0 < 2^31, y1 <= 2^31), x y = x0 y0 + (x0 y1 + x1 y0) 2^31 + x1 y1 2^62, and
+// 2^62 = 2 (mod P2). A product (or a sum of two) is therefore a low column x0 y0 + (2 x1) y1
+// and a middle column m = x0 y1 + x1 y0, each a chain of 32 x 32 -> 64 multiply-adds with no
+// carries between words, and the middle column enters as
+//     m 2^31 = (m >> 30) 2^61 + (m mod 2^30) 2^31 = (m >> 30) + (m mod 2^30) 2^31  (mod P2).
+struct Sp {
+    u32 x0, x1, x1d;  // x0, x1 and 2 x1 of a lazy word
 };
-// x * y for x, y < 2^62 as 128 bits: four 32x32->64 multiply-adds.
-__device__ __forceinline__ W128 mul64(u64 x, u64 y) {
-    u32 x0 = (u32)x, x1 = (u32)(x >> 32), y0 = (u32)y, y1 = (u32)(y >> 32);
-    u64 t = (u64)x0 * y0;
-    u64 u = (u64)x0 * y1 + (t >> 32);
-    u64 v = (u64)x1 * y0 + u;
-    u64 w = (u64)x1 * y1 + (v >> 32);
-    return W128{(v << 32) | (u32)t, w};
+// A canonical constant with both parts split at bit 31 (re = r1 2^31 + r0, im likewise).
+struct alignas(16) S2 {
+    u32 r0, r1, i0, i1;
+};
+
+MM_HD Sp spl(u64 x) {
+    u32 x1 = (u32)(x >> 31);
+    return Sp{(u32)x & P1, x1, x1 << 1};
 }
-// P mod P2 (lazy) for P < 2^124.5: (P & P2) + (P >> 61), then a fold.
-__device__ __forceinline__ u64 red2w(u64 lo, u64 hi) { return f2((lo & P2) + ((hi << 3) | (lo >> 61))); }
-__device__ __forceinline__ u64 red2p(W128 p) { return red2w(p.lo, p.hi); }
-// a + b for two products each below 2^123.01 (one factor canonical).
-__device__ __forceinline__ u64 red2sum(W128 a, W128 b) {
-    u128 s = (((u128)a.hi << 64) | a.lo) + (((u128)b.hi << 64) | b.lo);
-    return red2w((u64)s, (u64)(s >> 64));
+MM_HD S2 spl_c(F2 c) { return S2{(u32)c.re & P1, (u32)(c.re >> 31), (u32)c.im & P1, (u32)(c.im >> 31)}; }
+MM_HD u64 mad(u32 a, u32 b, u64 c) { return (u64)a * b + c; }
+// 2^31 held in a scalar register: a multiply-add by it is one instruction where a 64-bit shift,
+// two masks and an add would be four.
+MM_HD u32 k31() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    u32 k;
+    asm("s_mov_b32 %0, 0x80000000" : "=s"(k));
+    return k;
+#else
+    return 0x80000000u;
+#endif
 }
-// x * y for a 32-bit x and a canonical y.
-__device__ __forceinline__ u64 mul32x2(u32 x, u64 y) {
-    u64 t = (u64)x * (u32)y;
-    u64 u = (u64)x * (u32)(y >> 32) + (t >> 32);  // P = u 2^32 + t_lo < 2^93
-    u64 lo = (u << 32) | (u32)t;
-    return f2((lo & P2) + (u >> 29));
+// acc + m 2^31 (mod P2), folded, for acc <= 2^63 + 2^62 and m < 2^64 - 2^34.
+MM_HD u64 comb(u64 acc, u64 m) { return f2(mad((u32)m & 0x3fffffffu, k31(), acc + (m >> 30))); }
+// a c + b d for lazy a, b and canonical c = c1 2^31 + c0, d = d1 2^31 + d0 (c1, d1 < 2^30):
+// low column < 2 (2^62 + 2^61), middle column < 2^63.
+MM_HD u64 dot2(Sp a, Sp b, u32 c0, u32 c1, u32 d0, u32 d1) {
+    u64 acc = mad(a.x0, c0, mad(a.x1d, c1, mad(b.x0, d0, mad(b.x1d, d1, 0))));
+    u64 m = mad(a.x0, c1, mad(a.x1, c0, mad(b.x0, d1, mad(b.x1, d0, 0))));
+    return comb(acc, m);
 }
+// x y for lazy x and y = y1 2^31 + y0 with y0 < 2^31, y1 <= 2^31 (y < 2^62 + 2^31):
+// low column <= 2^62 + 2^62, middle column < 2^62 + 2^61.
+MM_HD u64 prod2(Sp x, u32 y0, u32 y1) { return comb(mad(x.x0, y0, mad(x.x1d, y1, 0)), mad(x.x0, y1, mad(x.x1, y0, 0))); }
 
 // ---- Lazy GF(M31): words in [0, P1] -------------------------------------------------------
-__device__ __forceinline__ u32 ad1(u32 a, u32 b) { u32 s = a + b; return min(s, s - P1); }
-__device__ __forceinline__ u32 sb1(u32 a, u32 b) { u32 d = a - b; return min(d, d + P1); }
-__device__ __forceinline__ u32 rot1(u32 x) { return ((x << 15) & P1) | (x >> 16); }  // x * 2^15
+MM_HD u32 umin(u32 a, u32 b) { return a < b ? a : b; }
+MM_HD u32 ad1(u32 a, u32 b) { u32 s = a + b; return umin(s, s - P1); }
+MM_HD u32 sb1(u32 a, u32 b) { u32 d = a - b; return umin(d, d + P1); }
+MM_HD u32 rot1(u32 x) { return ((x << 15) & P1) | (x >> 16); }  // x * 2^15
 // P mod P1 into [0, P1] for P <= 2 P1^2.
-__device__ __forceinline__ u32 red1w(u64 p) {
+MM_HD u32 red1w(u64 p) {
     u32 a = (u32)p & P1, b = (u32)(p >> 31);
     u32 s = a + (b & P1) + (b >> 31);
-    return min(s, s - P1);
+    return umin(s, s - P1);
 }
-__device__ __forceinline__ u32 canon1(u32 x) { return x == P1 ? 0 : x; }
+MM_HD u32 canon1(u32 x) { return x == P1 ? 0 : x; }
 
 // ---- Complex elements and per-field operations ----------------------------------------------
 struct Z1 {
@@ -92,73 +110,77 @@ struct Op1 {
     typedef u32 B;
     typedef Z1 Z;
     typedef F1 C;  // canonical constant
-    static __device__ __forceinline__ B ad(B a, B b) { return ad1(a, b); }
-    static __device__ __forceinline__ B sb(B a, B b) { return sb1(a, b); }
-    static __device__ __forceinline__ B rot(B a) { return rot1(a); }
+    static MM_HD B ad(B a, B b) { return ad1(a, b); }
+    static MM_HD B sb(B a, B b) { return sb1(a, b); }
+    static MM_HD B rot(B a) { return rot1(a); }
     // x * w, w canonical.
-    static __device__ __forceinline__ Z mul(Z x, C w) {
+    static MM_HD Z mul(Z x, C w) {
         u64 re = (u64)x.re * w.re + (u64)x.im * (P1 - w.im);
         u64 im = (u64)x.re * w.im + (u64)x.im * w.re;
         return Z{red1w(re), red1w(im)};
     }
-    static __device__ __forceinline__ Z sqr(Z x) {
+    static MM_HD Z sqr(Z x) {
         u64 re = (u64)x.re * x.re + (u64)x.im * (P1 - x.im);
         u64 im = 2 * ((u64)x.re * x.im);
         return Z{red1w(re), red1w(im)};
     }
-    static __device__ __forceinline__ Z scale(Z x, u32 c) { return Z{red1w((u64)x.re * c), red1w((u64)x.im * c)}; }
+    static MM_HD Z scale(Z x, u32 c) { return Z{red1w((u64)x.re * c), red1w((u64)x.im * c)}; }
     // A real 32-bit value times a canonical weight.
-    static __device__ __forceinline__ Z real_mul(u32 v, C w) {
+    static MM_HD Z real_mul(u32 v, C w) {
         u32 m = mod1(v);
         return Z{red1w((u64)m * w.re), red1w((u64)m * w.im)};
     }
-    static __device__ __forceinline__ Z conj(Z x) { return Z{x.re, sb1(0, x.im)}; }
-    static __device__ __forceinline__ C canon(Z x) { return C{canon1(x.re), canon1(x.im)}; }
+    static MM_HD Z conj(Z x) { return Z{x.re, sb1(0, x.im)}; }
 };
 
 struct Op2 {
     typedef u64 B;
     typedef Z2 Z;
-    typedef F2 C;
-    static __device__ __forceinline__ B ad(B a, B b) { return ad2(a, b); }
-    static __device__ __forceinline__ B sb(B a, B b) { return sb2(a, b); }
-    static __device__ __forceinline__ B rot(B a) { return rot2(a); }
-    static __device__ __forceinline__ Z mul(Z x, C w) {
-        return Z{red2sum(mul64(x.re, w.re), mul64(x.im, P2 - w.im)), red2sum(mul64(x.re, w.im), mul64(x.im, w.re))};
+    typedef S2 C;  // canonical constant, split
+    static MM_HD B ad(B a, B b) { return ad2(a, b); }
+    static MM_HD B sb(B a, B b) { return sb2(a, b); }
+    static MM_HD B rot(B a) { return rot2(a); }
+    // x * w: re = x.re w.re + x.im (P2 - w.im), im = x.re w.im + x.im w.re.
+    static MM_HD Z mul(Z x, C w) {
+        Sp a = spl(x.re), b = spl(x.im);
+        u32 n0 = P1 - w.i0, n1 = 0x3fffffffu - w.i1;  // P2 - w.im, split
+        return Z{dot2(a, b, w.r0, w.r1, n0, n1), dot2(a, b, w.i0, w.i1, w.r0, w.r1)};
     }
-    static __device__ __forceinline__ Z sqr(Z x) {
+    static MM_HD Z mul(Z x, F2 w) { return mul(x, spl_c(w)); }
+    // (re + i im)^2 = (re + im)(re - im) + i re (2 im).
+    static MM_HD Z sqr(Z x) {
         u64 s = ad2(x.re, x.im), d = sb2(x.re, x.im);
-        u64 m = red2p(mul64(x.re, x.im));
-        return Z{red2p(mul64(s, d)), f2(m + m)};
+        u64 y = x.im << 1;  // < 2^62 + 13
+        return Z{prod2(spl(s), (u32)d & P1, (u32)(d >> 31)), prod2(spl(x.re), (u32)y & P1, (u32)(y >> 31))};
     }
-    static __device__ __forceinline__ Z scale(Z x, u32 c) {
+    static MM_HD Z scale(Z x, u32 c) {
         // c <= 4: x c < 2^63.1.
         return Z{f2((u64)(u32)x.re * c + (((u64)(u32)(x.re >> 32) * c) << 32)),
                  f2((u64)(u32)x.im * c + (((u64)(u32)(x.im >> 32) * c) << 32))};
     }
-    static __device__ __forceinline__ Z real_mul(u32 v, C w) { return Z{mul32x2(v, w.re), mul32x2(v, w.im)}; }
-    static __device__ __forceinline__ Z conj(Z x) { return Z{x.re, sb2(0, x.im)}; }
-    static __device__ __forceinline__ C canon(Z x) { return C{canon2(x.re), canon2(x.im)}; }
+    // A real 32-bit value times a canonical weight: v w = v w0 + (v w1) 2^31.
+    static MM_HD Z real_mul(u32 v, C w) { return Z{comb(mad(v, w.r0, 0), mad(v, w.r1, 0)), comb(mad(v, w.i0, 0), mad(v, w.i1, 0))}; }
+    static MM_HD Z conj(Z x) { return Z{x.re, sb2(0, x.im)}; }
 };
 
-template <class O> __device__ __forceinline__ typename O::Z zadd(typename O::Z a, typename O::Z b) {
+template <class O> MM_HD typename O::Z zadd(typename O::Z a, typename O::Z b) {
     return {O::ad(a.re, b.re), O::ad(a.im, b.im)};
 }
-template <class O> __device__ __forceinline__ typename O::Z zsub(typename O::Z a, typename O::Z b) {
+template <class O> MM_HD typename O::Z zsub(typename O::Z a, typename O::Z b) {
     return {O::sb(a.re, b.re), O::sb(a.im, b.im)};
 }
 // (a - b) * i
-template <class O> __device__ __forceinline__ typename O::Z zsub_i(typename O::Z a, typename O::Z b) {
+template <class O> MM_HD typename O::Z zsub_i(typename O::Z a, typename O::Z b) {
     return {O::sb(b.im, a.im), O::sb(a.re, b.re)};
 }
 // (a - b) * (-i)
-template <class O> __device__ __forceinline__ typename O::Z zsub_mi(typename O::Z a, typename O::Z b) {
+template <class O> MM_HD typename O::Z zsub_mi(typename O::Z a, typename O::Z b) {
     return {O::sb(a.im, b.im), O::sb(b.re, a.re)};
 }
 
 // ---- Butterflies over a thread's slots ------------------------------------------------------
 // DIF radix 8 on v[o], v[o+1], ..., v[o+7]: output slot m holds frequency rev3(m).
-template <class O> __device__ __forceinline__ void dif8(typename O::Z* v) {
+template <class O> MM_HD void dif8(typename O::Z* v) {
     typedef typename O::Z Z;
     Z a0 = zadd<O>(v[0], v[4]), a1 = zadd<O>(v[1], v[5]), a2 = zadd<O>(v[2], v[6]), a3 = zadd<O>(v[3], v[7]);
     Z a4 = zsub<O>(v[0], v[4]);
@@ -178,7 +200,7 @@ template <class O> __device__ __forceinline__ void dif8(typename O::Z* v) {
     v[6] = zadd<O>(b6, b7);
     v[7] = zsub<O>(b6, b7);
 }
-template <class O> __device__ __forceinline__ void dif4(typename O::Z* v) {
+template <class O> MM_HD void dif4(typename O::Z* v) {
     typedef typename O::Z Z;
     Z a0 = zadd<O>(v[0], v[2]), a2 = zsub<O>(v[0], v[2]), a1 = zadd<O>(v[1], v[3]), a3 = zsub_i<O>(v[1], v[3]);
     v[0] = zadd<O>(a0, a1);
@@ -186,13 +208,13 @@ template <class O> __device__ __forceinline__ void dif4(typename O::Z* v) {
     v[2] = zadd<O>(a2, a3);
     v[3] = zsub<O>(a2, a3);
 }
-template <class O> __device__ __forceinline__ void dif2(typename O::Z* v) {
+template <class O> MM_HD void dif2(typename O::Z* v) {
     typename O::Z a = v[0], b = v[1];
     v[0] = zadd<O>(a, b);
     v[1] = zsub<O>(a, b);
 }
 // DIT radix 8 (inverse roots): input slot m holds frequency rev3(m), output natural.
-template <class O> __device__ __forceinline__ void dit8(typename O::Z* v) {
+template <class O> MM_HD void dit8(typename O::Z* v) {
     typedef typename O::Z Z;
     Z b0 = zadd<O>(v[0], v[1]), b1 = zsub<O>(v[0], v[1]), b2 = zadd<O>(v[2], v[3]), b3 = zsub<O>(v[2], v[3]);
     Z b4 = zadd<O>(v[4], v[5]), b5 = zsub<O>(v[4], v[5]), b6 = zadd<O>(v[6], v[7]), b7 = zsub<O>(v[6], v[7]);
@@ -213,7 +235,7 @@ template <class O> __device__ __forceinline__ void dit8(typename O::Z* v) {
     v[3] = Z{O::ad(c3.re, u3), O::sb(c3.im, w3)};
     v[7] = Z{O::sb(c3.re, u3), O::ad(c3.im, w3)};
 }
-template <class O> __device__ __forceinline__ void dit4(typename O::Z* v) {
+template <class O> MM_HD void dit4(typename O::Z* v) {
     typedef typename O::Z Z;
     Z b0 = zadd<O>(v[0], v[1]), b1 = zsub<O>(v[0], v[1]), b2 = zadd<O>(v[2], v[3]), b3 = zsub<O>(v[2], v[3]);
     v[0] = zadd<O>(b0, b2);
@@ -221,7 +243,7 @@ template <class O> __device__ __forceinline__ void dit4(typename O::Z* v) {
     v[1] = Z{O::ad(b1.re, b3.im), O::sb(b1.im, b3.re)};
     v[3] = Z{O::sb(b1.re, b3.im), O::ad(b1.im, b3.re)};
 }
-template <class O> __device__ __forceinline__ void dit2(typename O::Z* v) { dif2<O>(v); }
+template <class O> MM_HD void dit2(typename O::Z* v) { dif2<O>(v); }
 
 // ---- Pass plan ---------------------------------------------------------------------------
 // DIF passes in order: radix r_p (2, 4 or 8; only the first may be below 8) and stride s_p,
@@ -257,7 +279,7 @@ struct Xbuf {
 };
 
 // Butterfly over the thread's slots for a pass of radix r.
-template <class O, bool DIF> __device__ __forceinline__ void bfly(typename O::Z* v, u32 r) {
+template <class O, bool DIF> MM_HD void bfly(typename O::Z* v, u32 r) {
     if (r == 8) {
         if (DIF) dif8<O>(v); else dit8<O>(v);
     } else if (r == 4) {
@@ -307,9 +329,24 @@ __device__ __forceinline__ u32 opaque_tid() {
     return t;
 }
 
+// Pins a value in registers at this point: the compiler may not sink its computation past
+// later barriers (where it would keep the larger inputs live instead).
+__device__ __forceinline__ void pin(u32& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void pin(u64& v) { asm volatile("" : "+v"(v)); }
+
 // Scheduling fence: keeps the compiler from interleaving phases (and hoisting their loads),
 // which would multiply live registers across the unrolled transform.
 #define MM_FENCE() __builtin_amdgcn_sched_barrier(0)
+// Loops over a thread's elements that read constants from memory fence after every
+// MM_GROUP elements: otherwise the scheduler issues all eight elements' loads at once and
+// holds their registers until use. With two workgroups per CU, other waves hide the latency.
+#ifndef MM_GROUP
+#define MM_GROUP 2
+#endif
+#define MM_GROUP_FENCE(e)                              \
+    do {                                               \
+        if (((e) + 1) % MM_GROUP == 0) MM_FENCE();     \
+    } while (0)
 
 // The pass plan of length 2^LOGN at compile time (the same plan as make_plan on the host).
 template <u32 LOGN> struct CPlan {
@@ -384,12 +421,13 @@ __device__ __forceinline__ void exchange(u32 t, bool act, Z1* v1, Z2* v2, const 
 // Full DIF on registers laid out for pass 0; ends laid out for the last pass (positions
 // 8t .. 8t+7). Threads t >= n / 8 hold no elements but join every barrier.
 template <u32 LOGN, bool TILED>
-__device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, const F2* tw2, const Xbuf& X) {
+__device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, const S2* tw2, const Xbuf& X) {
     typedef CPlan<LOGN> P;
     const bool act = t < P::T;
 #pragma unroll
     for (u32 p = 0; p < P::npass; p++) {
         const u32 r = P::radix(p), ls = P::lstride(p);
+        t = opaque_tid();
         if (act) {
             bfly<Op1, true>(v1, r);
             twiddle<Op1>(v1, tw1 + P::tw_off(p), t, r, ls, P::T);
@@ -405,13 +443,14 @@ __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, c
 
 // Full DIT from the last pass's layout back to pass 0's (natural positions).
 template <u32 LOGN, bool TILED>
-__device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, const F2* itw2, const Xbuf& X) {
+__device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, const S2* itw2, const Xbuf& X) {
     typedef CPlan<LOGN> P;
     const bool act = t < P::T;
 #pragma unroll
     for (u32 q = 0; q < P::npass; q++) {
         const u32 p = P::npass - 1 - q;
         const u32 r = P::radix(p), ls = P::lstride(p);
+        t = opaque_tid();
         if (act) {
             twiddle<Op1>(v1, itw1 + P::tw_off(p), t, r, ls, P::T);
             bfly<Op1, false>(v1, r);
@@ -427,20 +466,24 @@ __device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, 
 
 // ---- Carry over registers ------------------------------------------------------------------
 // Thread t holds coefficients c[0..E) of positions E t .. E t + E - 1 (all blockDim threads,
-// count = blockDim * E). Writes the limbs of sum c_q 2^(32q) + cin to l and returns the carry
-// out of the top. Carry lookahead as in block_carry: neighbour high parts by shuffle/LDS, then
-// 0/1 carries by generate/propagate ballots over lanes and waves. sc: 2 * waves + 2 words.
-template <int E, class Coef>
-__device__ __forceinline__ u64 carry_regs(u32 t, Coef coef, u32* l, u64* sc, u64 cin) {
-    const u32 nt = blockDim.x, lane = t & 63, wave = t >> 6, nw = nt >> 6;
+// count = blockDim * E). carry_local sums them into limbs l with the local carry and returns the
+// high part (sum c_i 2^(32 i) >> 32 E); carry_fix adds the neighbours' high parts and resolves
+// the remaining 0/1 carries across the block: generate/propagate ballots over lanes, then over
+// waves. Together they write the limbs of sum c_q 2^(32q) + cin and return the carry out of
+// the top. sc: 2 * waves + 2 words.
+template <int E, class Coef> __device__ __forceinline__ u64 carry_local(Coef coef, u32* l) {
     u128 acc = 0;
 #pragma unroll
     for (int e = 0; e < E; e++) {
         acc += coef(e);
         l[e] = (u32)acc;
         acc >>= 32;
+        if ((e & 3) == 3) MM_FENCE();  // coefficient reads four at a time
     }
-    const u64 hi = (u64)acc;
+    return (u64)acc;
+}
+template <int E> __device__ __forceinline__ u64 carry_fix(u32 t, u32* l, u64 hi, u64* sc, u64 cin) {
+    const u32 nt = blockDim.x, lane = t & 63, wave = t >> 6, nw = nt >> 6;
     const u64 up = __shfl_up(hi, 1, 64);
     __syncthreads();
     if (lane == 63) sc[wave] = hi;
@@ -476,6 +519,10 @@ __device__ __forceinline__ u64 carry_regs(u32 t, Coef coef, u32* l, u64* sc, u64
     if (t == owner) sc[2 * nw] = hi + k + bit;
     __syncthreads();
     return sc[2 * nw];
+}
+template <int E, class Coef> __device__ __forceinline__ u64 carry_regs(u32 t, Coef coef, u32* l, u64* sc, u64 cin) {
+    const u64 hi = carry_local<E>(coef, l);
+    return carry_fix<E>(t, l, hi, sc, cin);
 }
 
 // Coefficient staging for the transposition: 3 words per coefficient in the exchange

@@ -10,9 +10,9 @@ namespace v2 {
 struct VTab {
     Plan P;
     const F1 *w1, *iw1;    // zeta^j, zeta^-j / n (canonical)
-    const F2 *w2, *iw2;
+    const S2 *w2, *iw2;    // the same in GF(M61^2), split at bit 31
     const F1 *tw1, *itw1;  // pass twiddles (layout of Plan)
-    const F2 *tw2, *itw2;
+    const S2 *tw2, *itw2;
 };
 
 struct VArgs {
@@ -20,6 +20,8 @@ struct VArgs {
     VTab ht;    // length h = n / 2 (weights unused)
     Slots s;
     u32 crt_inv;
+    const F1* iwc1;  // zeta^-j c / n for the batch's base c: a 1 bit's factor folded into the
+    const S2* iwc2;  // unweighting (c IDFT(X) = IDFT(c X))
 };
 
 // LDS bytes of the LDS-resident layout: exchange buffer (24 n), T (8 k), carry scratch.
@@ -28,10 +30,13 @@ __host__ __device__ inline uint64_t v2_lds_resident(uint32_t n, uint32_t k, uint
 }
 
 // One step's squaring: T = c x^2 into Tl (2k limbs); returns nonzero on a carry out.
+// iw1, iw2: the unweighting, zeta^-j / n or, on a 1 bit, zeta^-j c / n.
 template <u32 LOGN, bool TILED>
-__device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, u64* sc, const u32* x, u32 c, u32 inv) {
+__device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, u64* sc, const u32* x, const F1* iw1,
+                                         const S2* iw2, u32 inv) {
     typedef CPlan<LOGN> P;
-    const u32 t = opaque_tid(), T = P::T, n = P::n;
+    u32 t = opaque_tid();  // refreshed per phase: no address held across phases
+    const u32 T = P::T, n = P::n;
     const u32 r0 = P::radix(0), ls0 = P::lstride(0);
     Z1 v1[8];
     Z2 v2[8];
@@ -41,18 +46,16 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         u32 xv = x[pos];
         v1[e] = Op1::real_mul(xv, V.w1[pos]);
         v2[e] = Op2::real_mul(xv, V.w2[pos]);
+        MM_GROUP_FENCE(e);
     }
     dif_regs<LOGN, TILED>(t, v1, v2, V.tw1, V.tw2, X);
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
         v1[e] = Op1::sqr(v1[e]);
         v2[e] = Op2::sqr(v2[e]);
-        if (c != 1) {
-            v1[e] = Op1::scale(v1[e], c);
-            v2[e] = Op2::scale(v2[e], c);
-        }
     }
     dit_regs<LOGN, TILED>(t, v1, v2, V.itw1, V.itw2, X);
+    t = opaque_tid();
     typedef Tiling<LOGN, TILED> G;
     u32 l[16];
     u64 top;
@@ -62,40 +65,51 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
 #pragma unroll
         for (u32 e = 0; e < 8; e++) {
             u32 pos = slot_pos(t, e, r0, ls0, T);
-            Z1 a = Op1::mul(v1[e], V.iw1[pos]);
-            Z2 b = Op2::mul(v2[e], V.iw2[pos]);
+            Z1 a = Op1::mul(v1[e], iw1[pos]);
+            Z2 b = Op2::mul(v2[e], iw2[pos]);
             cput(C, pos, crt_lazy(a.re, b.re, inv));
             cput(C, pos + n, crt_lazy(a.im, b.im, inv));
+            MM_GROUP_FENCE(e);
         }
         __syncthreads();
+        t = opaque_tid();
         top = carry_regs<16>(t, [&](int i) { return cget(C, 16 * t + i); }, l, sc, 0);
     } else {
-        // The tile holds 2 * tile coefficients: the real parts (coefficients 0 .. n-1) go
-        // through it first, then the imaginary parts (n .. 2n-1), each thread keeping its 16.
+        // The tile holds 2 * tile = n coefficients: the real parts (coefficients 0 .. n-1) go
+        // through it and are carried first, then the imaginary parts (n .. 2n-1) with the
+        // carry out of the low half; only the imaginary residues are held in between.
         static_assert(2 * G::tile == n, "two coefficient chunks");
         const Cbuf C = cbuf(X, G::tile);
-        u128 cre[8], cim[8];
+        u32 im1[8];
+        u64 im2[8];
+        __syncthreads();
 #pragma unroll
         for (u32 e = 0; e < 8; e++) {
             u32 pos = slot_pos(t, e, r0, ls0, T);
-            Z1 a = Op1::mul(v1[e], V.iw1[pos]);
-            Z2 b = Op2::mul(v2[e], V.iw2[pos]);
-            cre[e] = crt_lazy(a.re, b.re, inv);
-            cim[e] = crt_lazy(a.im, b.im, inv);
+            Z1 a = Op1::mul(v1[e], iw1[pos]);
+            Z2 b = Op2::mul(v2[e], iw2[pos]);
+            cput(C, pos, crt_lazy(a.re, b.re, inv));
+            im1[e] = a.im;
+            im2[e] = b.im;
+            pin(im1[e]);
+            pin(im2[e]);
+            MM_GROUP_FENCE(e);
         }
-        u128 mine[16];
+        __syncthreads();
+        u32 lo[8];
+        t = opaque_tid();
+        const u64 mid = carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i); }, lo, sc, 0);
 #pragma unroll
-        for (u32 c = 0; c < 2; c++) {
-            __syncthreads();
+        for (u32 i = 0; i < 8; i++) Tl[8 * t + i] = lo[i];
 #pragma unroll
-            for (u32 e = 0; e < 8; e++) cput(C, slot_pos(t, e, r0, ls0, T), c == 0 ? cre[e] : cim[e]);
-            __syncthreads();
-            if ((16 * t) / n == c) {
+        for (u32 e = 0; e < 8; e++) cput(C, slot_pos(t, e, r0, ls0, T), crt_lazy(im1[e], im2[e], inv));
+        __syncthreads();
+        u32 hi[8];
+        t = opaque_tid();
+        const u64 top = carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i); }, hi, sc, mid);
 #pragma unroll
-                for (u32 i = 0; i < 16; i++) mine[i] = cget(C, 16 * t + i - c * n);
-            }
-        }
-        top = carry_regs<16>(t, [&](int i) { return mine[i]; }, l, sc, 0);
+        for (u32 i = 0; i < 8; i++) Tl[n + 8 * t + i] = hi[i];
+        return top != 0;
     }
 #pragma unroll
     for (u32 i = 0; i < 16; i++) Tl[16 * t + i] = l[i];
@@ -112,7 +126,8 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     const VTab& H = A.ht;
     typedef CPlan<LOGN> P;
     typedef CPlan<LOGN - 1> PH;
-    const u32 t = opaque_tid(), T = P::T, n = P::n, k = n, h = n / 2, Th = PH::T;
+    u32 t = opaque_tid();  // refreshed per phase: no address held across phases
+    const u32 T = P::T, n = P::n, k = n, h = n / 2, Th = PH::T;
     const u32 r0 = P::radix(0), ls0 = P::lstride(0);
     const u32 inv = A.crt_inv;
     // Staging for k = n coefficients: the exchange buffer (one tile) holds 2 * tile of them.
@@ -127,25 +142,32 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         u32 tv = Tl[pos];
         v1[e] = Op1::real_mul(tv, V.w1[pos]);
         v2[e] = Op2::real_mul(tv, V.w2[pos]);
+        MM_GROUP_FENCE(e);
     }
     dif_regs<LOGN, TILED>(t, v1, v2, V.tw1, V.tw2, X);
+    t = opaque_tid();
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
         v1[e] = Op1::mul(v1[e], np1[8 * t + e]);
         v2[e] = Op2::mul(v2[e], np2[8 * t + e]);
+        MM_GROUP_FENCE(e);
     }
     dit_regs<LOGN, TILED>(t, v1, v2, V.itw1, V.itw2, X);
     __syncthreads();
+    t = opaque_tid();
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
         u32 pos = slot_pos(t, e, r0, ls0, T);
         Z1 a = Op1::mul(v1[e], V.iw1[pos]);
         Z2 b = Op2::mul(v2[e], V.iw2[pos]);
         cput(C, pos, crt_lazy(a.re, b.re, inv));
+        MM_GROUP_FENCE(e);
     }
     __syncthreads();
+    t = opaque_tid();
     u32 m[8];
     carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i); }, m, sc, 0);
+    t = opaque_tid();
     // m packed as z_j = m_2j + i m_2j+1 at half-length positions j = 4t .. 4t+3.
 #pragma unroll
     for (u32 i = 0; i < 4; i++) {
@@ -154,9 +176,11 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         X.z2[q] = Z2{m[2 * i], m[2 * i + 1]};
     }
     __syncthreads();
+    t = opaque_tid();
     const bool hact = t < Th;
     if (hact) get(X, v1, v2, t, PH::radix(0), PH::lstride(0), Th);
     dif_regs<LOGN - 1, TILED>(t, v1, v2, H.tw1, H.tw2, X);
+    t = opaque_tid();
     // Pointwise with the partner position: out_p = (z_p + conj z_q) A_p + (z_p - conj z_q) B_p.
     __syncthreads();
     if (hact) {
@@ -168,6 +192,7 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         }
     }
     __syncthreads();
+    t = opaque_tid();
     if (hact) {
 #pragma unroll
         for (u32 e = 0; e < 8; e++) {
@@ -178,9 +203,12 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
             Z2 e2 = zadd<Op2>(v2[e], zq2), d2 = zsub<Op2>(v2[e], zq2);
             v1[e] = zadd<Op1>(Op1::mul(e1, ab1[p]), Op1::mul(d1, ab1[h + p]));
             v2[e] = zadd<Op2>(Op2::mul(e2, ab2[p]), Op2::mul(d2, ab2[h + p]));
+            MM_GROUP_FENCE(e);
         }
     }
+    t = opaque_tid();
     dit_regs<LOGN - 1, TILED>(t, v1, v2, H.itw1, H.itw2, X);
+    t = opaque_tid();
     // Cyclic product element j carries coefficients 2j (re) and 2j + 1 (im).
     __syncthreads();
     if (hact) {
@@ -189,22 +217,27 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
             u32 j = slot_pos(t, e, PH::radix(0), PH::lstride(0), Th);
             cput(C, 2 * j, crt_lazy(v1[e].re, v2[e].re, inv));
             cput(C, 2 * j + 1, crt_lazy(v1[e].im, v2[e].im, inv));
+            MM_GROUP_FENCE(e);
         }
     }
     __syncthreads();
+    t = opaque_tid();
     // G = (cyclic + T mod R) mod (R - 1), x = floor(T / R) + G.
     u32 xs[8];
     u64 cc = carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i) + Tl[8 * t + i]; }, xs, sc, 0);
+#pragma nounroll
     for (int guard = 0; cc != 0 && guard < 4; guard++) {
         u32 prev[8];
 #pragma unroll
         for (u32 i = 0; i < 8; i++) prev[i] = xs[i];
         cc = carry_regs<8>(t, [&](int i) { return (u128)prev[i]; }, xs, sc, cc);
     }
+    t = opaque_tid();
     u32 lo[8];
 #pragma unroll
     for (u32 i = 0; i < 8; i++) lo[i] = xs[i];
     u64 top = carry_regs<8>(t, [&](int i) { return (u128)lo[i] + Tl[k + 8 * t + i]; }, xs, sc, 0);
+    t = opaque_tid();
 #pragma unroll
     for (u32 i = 0; i < 8; i++) x[8 * t + i] = xs[i];
     return (top != 0) | (cc != 0);
