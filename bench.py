@@ -5,7 +5,8 @@ Workload (the reference's score workload, montmul/src/score.rs:26-28, 45-61, 93-
 odd 100,000-bit moduli from GMP's generator seeded with 2026 (top bit and bit 0 set),
 k = 4,096 limbs, L = 8,192, base 3, exponent = the top 2,000 bits of N - 1 in every slot.
 A bench "step" is one pass of the hot path over the batch: re-arm every slot (load 1, set the
-exponent) and run(2,000), i.e. slots x 2,000 modular squarings. Warm-up passes run(64).
+exponent) and run(2,000), i.e. slots x 2,000 modular squarings. Warm-up passes are the same
+run(2,000), so every step-kernel launch of a run with a fixed slot count does the same work.
 
 The slot count is the best of 256 x {1, 2, 4, 8, 16} (CUs x {1, 2, 4, 8, 16}, as the
 reference's candidates, main.rs:199-216) at 200-bit exponents, unless --slots is given. The
@@ -53,12 +54,12 @@ def mad(v):
 
 
 def measure(mm, hip, moduli, exps, slots, ebits, repeats, warmup):
-    """Warm-up run(64) passes, then `repeats` timed re-armed run(ebits) passes."""
+    """`warmup` untimed and `repeats` timed re-armed run(ebits) passes."""
     b = mm.ModBatch(hip, moduli[:slots], 3)
     ex = exps[:slots]
     for _ in range(warmup):
         arm(b, ex)
-        b.run(64)
+        b.run(ebits)
     rates, secs, kms = [], [], []
     for _ in range(repeats):
         arm(b, ex)
@@ -128,13 +129,16 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5, help="timed passes (repeats of run(ebits))")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed run(64) passes")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes before the timed ones")
     ap.add_argument("--slots", type=int, default=0, help="slot count (0: best of 256 x {1,2,4,8,16})")
     ap.add_argument("--bits", type=int, default=100_000)
     ap.add_argument("--ebits", type=int, default=2000)
     ap.add_argument("--sweep-ebits", type=int, default=200)
     ap.add_argument("--modes", default="global,lds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (4 times): the reference kernel for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (slots would shard "
@@ -221,6 +225,16 @@ def main():
                                 "kind": "port", "sample": "oracle Gmp restatement (gmp.rs closed-form REDC): " + cb["gmp_sample"],
                                 "cpu_ntt_value": round(cb["cpu_ntt"], 1),
                                 "cpu_ntt_sample": "oracle Cpu restatement (cpu.rs NTT, wraparound): " + cb["cpu_sample"]}
+    if args.copy_calibration:
+        import torch
+
+        nbytes = args.copy_calibration << 20
+        src = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        line["copy_calibration"] = {"bytes_read": nbytes, "bytes_written": nbytes, "launches": 4}
     print(json.dumps(line), flush=True)
 
 

@@ -134,6 +134,41 @@ int mm_device_name(void *self, char *buf, size_t len);
 int mm_last_run_kernel_ms(void *state, double *ms, uint32_t *launches);
 
 /* ------------------------------------------------------------------------------------ */
+/* The transform layer of a Hip table (trait Transform, transform.rs:22-46; Gpu's impl,      */
+/* gpu/mod.rs:286-319). A buffer holds `count` vectors of length L = len: L/2 elements of  */
+/* GF(M31^2) x GF(M61^2) each, element j carrying coefficient j (real part) and j + L/2     */
+/* (imaginary part); forward, pointwise and inverse give products modulo X^L + 1, exact      */
+/* while coefficients stay below Q = M31 * M61. Spectra are in the reference's order.       */
+/* Functions return 0 or nonzero with mm_last_error().                                      */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct mm_tbuf mm_tbuf;
+typedef struct mm_tscales mm_tscales;
+
+/* Transform::upload (transform.rs:28): len * count coefficients, row-major, untransformed. */
+int mm_t_upload(void *self, const uint32_t *coeffs, uint32_t len, uint32_t count, mm_tbuf **out);
+/* Gpu::load_forward (gpu/mod.rs:218-225): weights and forward-transforms `count` strings of
+ * k <= len/2 limbs each (row-major), zero above. */
+int mm_t_load_forward(void *self, const uint32_t *limbs, uint32_t k, uint32_t len, uint32_t count, mm_tbuf **out);
+/* Transform::download (transform.rs:30): len * count exact coefficients in [0, Q), written as
+ * two 64-bit words (low, high) each. */
+int mm_t_download(const mm_tbuf *b, uint64_t *out);
+/* Gpu::download_raw (gpu/mod.rs:212-215): the elements without CRT, as (re, im) pairs:
+ * len/2 * count pairs of 32-bit words (M31) and of 64-bit words (M61). */
+int mm_t_download_raw(const mm_tbuf *b, uint32_t *f1, uint64_t *f2);
+/* Transform::upload_scales (transform.rs:31). */
+int mm_t_upload_scales(void *self, const uint32_t *scales, uint32_t count, mm_tscales **out);
+/* Transform::forward, inverse (transform.rs:33-34), in place; len and count must match. */
+int mm_t_forward(mm_tbuf *b, uint32_t len, uint32_t count);
+int mm_t_inverse(mm_tbuf *b, uint32_t len, uint32_t count);
+/* Transform::pointwise (transform.rs:35-45): out[i] = scale[i] a[i] (.) b[i]; scales may be NULL. */
+int mm_t_pointwise(mm_tbuf *out, const mm_tbuf *a, const mm_tbuf *b, const mm_tscales *scales, uint32_t len,
+                   uint32_t count);
+/* Drops of a buffer or a scale vector (device memory freed). */
+void mm_t_free(mm_tbuf *b);
+void mm_t_free_scales(mm_tscales *s);
+
+/* ------------------------------------------------------------------------------------ */
 /* Layer 2: the host API over any backend table.                                          */
 /* ------------------------------------------------------------------------------------ */
 

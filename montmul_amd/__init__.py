@@ -82,7 +82,7 @@ def _load_lib():
     if not os.path.exists(LIB_PATH):
         raise MontmulError(f"{LIB_PATH} is not built; run `python -m montmul_amd.build`")
     L = C.CDLL(LIB_PATH)
-    u32p, vp = C.POINTER(C.c_uint32), C.c_void_p
+    u32p, u64p, vp = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.c_void_p
     sig = {
         "mm_device_count": (C.c_int, []),
         "mm_hip_open": (C.c_int, [C.POINTER(Options), C.POINTER(BackendTable)]),
@@ -119,6 +119,16 @@ def _load_lib():
                                 C.POINTER(C.c_int8), C.c_uint32, C.c_uint64, C.POINTER(C.c_uint8)]),
         "mm_score_workload": (C.c_int, [C.c_uint32, C.c_size_t, C.c_uint32, C.c_uint64, u32p, u32p]),
         "mm_modbatch_check_pow": (C.c_int, [vp, C.c_size_t, C.c_uint32, u32p, C.c_size_t]),
+        "mm_t_upload": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
+        "mm_t_load_forward": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
+        "mm_t_download": (C.c_int, [vp, u64p]),
+        "mm_t_download_raw": (C.c_int, [vp, u32p, u64p]),
+        "mm_t_upload_scales": (C.c_int, [vp, u32p, C.c_uint32, C.POINTER(vp)]),
+        "mm_t_forward": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+        "mm_t_inverse": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+        "mm_t_pointwise": (C.c_int, [vp, vp, vp, vp, C.c_uint32, C.c_uint32]),
+        "mm_t_free": (None, [vp]),
+        "mm_t_free_scales": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -264,6 +274,124 @@ class Hip(Backend):
         ms, n = C.c_double(), C.c_uint32()
         lib.mm_last_run_kernel_ms(batch.state, C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+
+def _hip_error():
+    return MontmulError((lib.mm_last_error() or b"").decode(errors="replace"))
+
+
+class TBuf:
+    """`count` transform vectors of length `len` in device memory (GpuBuf, gpu/mod.rs:228-236)."""
+
+    def __init__(self, handle, length, count):
+        self._h, self.len, self.count = handle, length, count
+
+    def free(self):
+        if self._h:
+            lib.mm_t_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class TScales:
+    """Per-vector scalars on the device (Transform::Scales, transform.rs:26)."""
+
+    def __init__(self, handle, count):
+        self._h, self.count = handle, count
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib.mm_t_free_scales(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def _u32_array(values, count):
+    """A ctypes uint32 array of `count` (at least 1) words from any sequence or numpy array."""
+    try:
+        import numpy as np
+    except ImportError:  # plain Python fallback for the host layer
+        arr = (C.c_uint32 * max(count, 1))()
+        for i, v in enumerate(values):
+            arr[i] = int(v) & 0xFFFFFFFF
+        return arr
+    a = np.zeros(max(count, 1), dtype=np.uint32)
+    if count:
+        v = values if isinstance(values, np.ndarray) else np.array([int(x) & 0xFFFFFFFF for x in values], dtype=np.uint64)
+        a[:count] = v.astype(np.uint64, copy=False) & 0xFFFFFFFF
+    return (C.c_uint32 * a.size).from_buffer(a)
+
+
+def _transform_methods(cls):
+    """The transform layer of a Hip table (trait Transform, transform.rs:22-46; Gpu's impl,
+    gpu/mod.rs:286-319, with load_forward and download_raw, gpu/mod.rs:212-225)."""
+
+    def upload(self, coeffs, length, count):
+        n = length * count
+        if len(coeffs) != n:
+            raise MontmulError(f"upload: {len(coeffs)} coefficients for {count} vectors of length {length}")
+        h = C.c_void_p()
+        if lib.mm_t_upload(self.table.self, _u32_array(coeffs, n), length, count, C.byref(h)) != 0:
+            raise _hip_error()
+        return TBuf(h.value, length, count)
+
+    def load_forward(self, limbs, k, length, count):
+        h = C.c_void_p()
+        if lib.mm_t_load_forward(self.table.self, _u32_array(limbs, k * count), k, length, count, C.byref(h)) != 0:
+            raise _hip_error()
+        return TBuf(h.value, length, count)
+
+    def download_words(self, buf):
+        """Exact coefficients as (low, high) 64-bit word pairs, len * count of them."""
+        out = (C.c_uint64 * (2 * buf.len * buf.count))()
+        if lib.mm_t_download(buf._h, out) != 0:
+            raise _hip_error()
+        return out
+
+    def download(self, buf):
+        w = self.download_words(buf)
+        return [w[2 * i] | (w[2 * i + 1] << 64) for i in range(buf.len * buf.count)]
+
+    def download_raw(self, buf):
+        """(f1, f2): the elements without CRT as flat (re, im) arrays (Gpu::download_raw)."""
+        el = buf.len // 2 * buf.count
+        f1, f2 = (C.c_uint32 * (2 * el))(), (C.c_uint64 * (2 * el))()
+        if lib.mm_t_download_raw(buf._h, f1, f2) != 0:
+            raise _hip_error()
+        return f1, f2
+
+    def upload_scales(self, scales):
+        h = C.c_void_p()
+        if lib.mm_t_upload_scales(self.table.self, _u32_array(scales, len(scales)), len(scales), C.byref(h)) != 0:
+            raise _hip_error()
+        return TScales(h.value, len(scales))
+
+    def forward(self, buf, length, count):
+        if lib.mm_t_forward(buf._h, length, count) != 0:
+            raise _hip_error()
+
+    def inverse(self, buf, length, count):
+        if lib.mm_t_inverse(buf._h, length, count) != 0:
+            raise _hip_error()
+
+    def pointwise(self, out, a, b, scales, length, count):
+        s = scales._h if scales is not None else None
+        if lib.mm_t_pointwise(out._h, a._h, b._h, s, length, count) != 0:
+            raise _hip_error()
+
+    for f in (upload, load_forward, download_words, download, download_raw, upload_scales, forward, inverse, pointwise):
+        setattr(cls, f.__name__, f)
+    return cls
+
+
+_transform_methods(Hip)
 
 
 def _check_run(r, msg):

@@ -138,6 +138,71 @@ __global__ void k_scale_tab(const F1* iw1, const v2::S2* iw2, uint32_t c, uint32
     o2[j] = v2::spl_c(F2{mul2(re, c), mul2(im, c)});
 }
 
+// ---- Transform layer (reference: trait Transform, transform.rs:22-46) ----------------------
+// `count` vectors of n = L/2 elements, vector v at f1 + v n and f2 + v n. Element j of a vector
+// holds coefficients j (real part) and j + n (imaginary part). One workgroup per vector for the
+// in-place transforms; grid-stride loops over all elements for the element-wise kernels.
+
+// Coefficients (L per vector, row-major) into elements (Cpu::upload, cpu.rs:353-364).
+__global__ void k_t_upload(const uint32_t* c, uint32_t n, uint64_t total, F1* f1, F2* f2) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = e / n, j = e % n;
+        const uint32_t re = c[2 * n * v + j], im = c[2 * n * v + n + j];
+        f1[e] = F1{mod1(re), mod1(im)};
+        f2[e] = F2{re, im};
+    }
+}
+
+// Weighted DIF in place: natural order in, bit-reversed out (forward1, cpu.rs:86-95).
+__global__ __launch_bounds__(1024) void k_t_forward(Tab t, F1* f1, F2* f2) {
+    F1* a1 = f1 + (size_t)blockIdx.x * t.n;
+    F2* a2 = f2 + (size_t)blockIdx.x * t.n;
+    for (uint32_t j = tid(); j < t.n; j += nthreads()) {
+        a1[j] = cmul(a1[j], t.w1[j]);
+        a2[j] = cmul(a2[j], t.w2[j]);
+    }
+    dif(a1, t, t.dtw1);
+    dif(a2, t, t.dtw2);
+}
+
+// DIT in place and the unweighting zeta^-j / n (inverse1, cpu.rs:96-105).
+__global__ __launch_bounds__(1024) void k_t_inverse(Tab t, F1* f1, F2* f2) {
+    F1* a1 = f1 + (size_t)blockIdx.x * t.n;
+    F2* a2 = f2 + (size_t)blockIdx.x * t.n;
+    dit(a1, t, t.itw1);
+    dit(a2, t, t.itw2);
+    unweight(a1, a2, t);
+}
+
+// out = scale[v] a (.) b (Cpu::pointwise, cpu.rs:397-406 with pointwise1, 131-140); no scales: 1.
+__global__ void k_t_pointwise(F1* o1, F2* o2, const F1* a1, const F2* a2, const F1* b1, const F2* b2, const uint32_t* scales,
+                              uint32_t n, uint64_t total) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        F1 r1 = cmul(a1[e], b1[e]);
+        F2 r2 = cmul(a2[e], b2[e]);
+        if (scales) {
+            const uint32_t c = scales[e / n];
+            r1 = cscale_small(r1, c);
+            r2 = cscale_small(r2, c);
+        }
+        o1[e] = r1;
+        o2[e] = r2;
+    }
+}
+
+// Exact coefficients in [0, Q) by CRT, two words (low, high) each (Cpu::download, cpu.rs:365-377).
+__global__ void k_t_download(const F1* f1, const F2* f2, uint32_t n, uint64_t total, uint32_t inv, uint64_t* out) {
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = e / n, j = e % n;
+        const u128 lo = crt(f1[e].re, f2[e].re, inv), hi = crt(f1[e].im, f2[e].im, inv);
+        const uint64_t q0 = 2 * n * v + j, q1 = q0 + n;
+        out[2 * q0] = (uint64_t)lo;
+        out[2 * q0 + 1] = (uint64_t)(lo >> 64);
+        out[2 * q1] = (uint64_t)hi;
+        out[2 * q1 + 1] = (uint64_t)(hi >> 64);
+    }
+}
+
 // Weighted forward transform of k-limb strings into out (n elements per vector), in place in
 // the output (reference: `t_load_forward`, lib.rs:1040-1049; used for N' and, without the
 // wraparound correction, N).
@@ -1014,6 +1079,187 @@ int mm_last_run_kernel_ms(void* st, double* ms, uint32_t* launches) {
     *ms = S->last_ms;
     *launches = S->last_launches;
     return 0;
+}
+
+}  // extern "C"
+
+// ---- Transform layer (trait Transform for Gpu, gpu/mod.rs:286-319; load_forward and
+// download_raw, gpu/mod.rs:212-225) ------------------------------------------------------------
+struct mm_tbuf {
+    Ctx* C;
+    uint32_t len, count;
+    F1* f1;
+    F2* f2;
+};
+struct mm_tscales {
+    Ctx* C;
+    uint32_t count;
+    uint32_t* s;
+};
+
+namespace {
+// Grid of a grid-stride element kernel: a few workgroups per CU.
+dim3 elem_grid(const Ctx* C, uint64_t total) {
+    const uint64_t want = (total + 255) / 256, cap = 8ull * (uint64_t)std::max(1, C->cus);
+    return dim3((uint32_t)std::max<uint64_t>(1, std::min(want, cap)));
+}
+int check_len(Ctx* C, uint32_t len) {
+    if (len < 8 || (len & (len - 1)) || len > (1u << 24)) return fail("unsupported transform length");
+    (void)C;
+    return 0;
+}
+int new_tbuf(Ctx* C, uint32_t len, uint32_t count, mm_tbuf** out) {
+    *out = nullptr;
+    if (check_len(C, len)) return 1;
+    if (count == 0) return fail("empty transform batch");
+    std::unique_ptr<mm_tbuf> b(new mm_tbuf{C, len, count, nullptr, nullptr});
+    const size_t el = (size_t)len / 2 * count;
+    if (dalloc(&b->f1, el) || dalloc(&b->f2, el)) {
+        hipFree(b->f1);
+        return 1;
+    }
+    *out = b.release();
+    return 0;
+}
+int match(const mm_tbuf* b, uint32_t len, uint32_t count) {
+    if (!b) return fail("null transform buffer");
+    if (b->len != len || b->count != count) return fail("transform buffer holds another length or count");
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int mm_t_upload(void* self, const uint32_t* coeffs, uint32_t len, uint32_t count, mm_tbuf** out) {
+    Ctx* C = (Ctx*)self;
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    if (new_tbuf(C, len, count, out)) return 1;
+    mm_tbuf* b = *out;
+    uint32_t* d = nullptr;
+    const size_t words = (size_t)len * count;
+    if (dalloc(&d, words)) { mm_t_free(b); *out = nullptr; return 1; }
+    hipError_t e = hipMemcpyAsync(d, coeffs, words * 4, hipMemcpyHostToDevice, C->stream);
+    const uint64_t total = (uint64_t)len / 2 * count;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_t_upload, elem_grid(C, total), dim3(256), 0, C->stream, d, len / 2, total, b->f1, b->f2);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+    hipFree(d);
+    if (e != hipSuccess) { mm_t_free(b); *out = nullptr; return hip_fail("upload", e); }
+    return 0;
+}
+
+int mm_t_load_forward(void* self, const uint32_t* limbs, uint32_t k, uint32_t len, uint32_t count, mm_tbuf** out) {
+    Ctx* C = (Ctx*)self;
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    if (check_len(C, len)) return 1;
+    if (k > len / 2) return fail("k exceeds L/2");
+    std::shared_ptr<DevTables> T;
+    if (get_tables(C, len / 2, &T)) return 1;
+    if (new_tbuf(C, len, count, out)) return 1;
+    mm_tbuf* b = *out;
+    uint32_t* d = nullptr;
+    const size_t words = (size_t)k * count;
+    if (dalloc(&d, std::max<size_t>(words, 1))) { mm_t_free(b); *out = nullptr; return 1; }
+    hipError_t e = words ? hipMemcpyAsync(d, limbs, words * 4, hipMemcpyHostToDevice, C->stream) : hipSuccess;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_load_forward, dim3(count), dim3(plan_threads(len / 2)), 0, C->stream, T->t, d, k, b->f1, b->f2);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+    hipFree(d);
+    if (e != hipSuccess) { mm_t_free(b); *out = nullptr; return hip_fail("load_forward", e); }
+    return 0;
+}
+
+int mm_t_download(const mm_tbuf* b, uint64_t* out) {
+    if (!b) return fail("null transform buffer");
+    Ctx* C = b->C;
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    std::shared_ptr<DevTables> T;
+    if (get_tables(C, b->len / 2, &T)) return 1;
+    uint64_t* d = nullptr;
+    const size_t words = 2ull * b->len * b->count;
+    if (dalloc(&d, words)) return 1;
+    const uint64_t total = (uint64_t)b->len / 2 * b->count;
+    hipLaunchKernelGGL(k_t_download, elem_grid(C, total), dim3(256), 0, C->stream, b->f1, b->f2, b->len / 2, total,
+                       T->t.crt_inv, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, words * 8, hipMemcpyDeviceToHost, C->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail("download", e);
+    return 0;
+}
+
+int mm_t_download_raw(const mm_tbuf* b, uint32_t* f1, uint64_t* f2) {
+    if (!b) return fail("null transform buffer");
+    Ctx* C = b->C;
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    const size_t el = (size_t)b->len / 2 * b->count;
+    HIPCHK(hipMemcpyAsync(f1, b->f1, el * sizeof(F1), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipMemcpyAsync(f2, b->f2, el * sizeof(F2), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
+int mm_t_upload_scales(void* self, const uint32_t* scales, uint32_t count, mm_tscales** out) {
+    Ctx* C = (Ctx*)self;
+    *out = nullptr;
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    std::unique_ptr<mm_tscales> s(new mm_tscales{C, count, nullptr});
+    if (dalloc(&s->s, std::max<uint32_t>(count, 1))) return 1;
+    if (count) {
+        hipError_t e = hipMemcpy(s->s, scales, (size_t)count * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { hipFree(s->s); return hip_fail("upload_scales", e); }
+    }
+    *out = s.release();
+    return 0;
+}
+
+static int transform_pass(mm_tbuf* b, uint32_t len, uint32_t count, bool fwd) {
+    if (match(b, len, count)) return 1;
+    Ctx* C = b->C;
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    std::shared_ptr<DevTables> T;
+    if (get_tables(C, len / 2, &T)) return 1;
+    if (fwd)
+        hipLaunchKernelGGL(k_t_forward, dim3(count), dim3(plan_threads(len / 2)), 0, C->stream, T->t, b->f1, b->f2);
+    else
+        hipLaunchKernelGGL(k_t_inverse, dim3(count), dim3(plan_threads(len / 2)), 0, C->stream, T->t, b->f1, b->f2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+int mm_t_forward(mm_tbuf* b, uint32_t len, uint32_t count) { return transform_pass(b, len, count, true); }
+int mm_t_inverse(mm_tbuf* b, uint32_t len, uint32_t count) { return transform_pass(b, len, count, false); }
+
+int mm_t_pointwise(mm_tbuf* out, const mm_tbuf* a, const mm_tbuf* b, const mm_tscales* s, uint32_t len, uint32_t count) {
+    if (match(out, len, count) || match(a, len, count) || match(b, len, count)) return 1;
+    if (s && s->count < count) return fail("fewer scales than vectors");
+    Ctx* C = out->C;
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    const uint64_t total = (uint64_t)len / 2 * count;
+    hipLaunchKernelGGL(k_t_pointwise, elem_grid(C, total), dim3(256), 0, C->stream, out->f1, out->f2, a->f1, a->f2, b->f1,
+                       b->f2, s ? s->s : nullptr, len / 2, total);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(C->stream));
+    return 0;
+}
+
+void mm_t_free(mm_tbuf* b) {
+    if (!b) return;
+    hipSetDevice(b->C->device);
+    hipFree(b->f1);
+    hipFree(b->f2);
+    delete b;
+}
+void mm_t_free_scales(mm_tscales* s) {
+    if (!s) return;
+    hipSetDevice(s->C->device);
+    hipFree(s->s);
+    delete s;
 }
 
 }  // extern "C"

@@ -113,6 +113,9 @@ struct Op1 {
     static MM_HD B ad(B a, B b) { return ad1(a, b); }
     static MM_HD B sb(B a, B b) { return sb1(a, b); }
     static MM_HD B rot(B a) { return rot1(a); }
+    // First-stage add/sub of a butterfly: GF(M31) words have no headroom, so these reduce.
+    static MM_HD B adL(B a, B b) { return ad1(a, b); }
+    static MM_HD B sbL(B a, B b) { return sb1(a, b); }
     // x * w, w canonical.
     static MM_HD Z mul(Z x, C w) {
         u64 re = (u64)x.re * w.re + (u64)x.im * (P1 - w.im);
@@ -140,6 +143,10 @@ struct Op2 {
     static MM_HD B ad(B a, B b) { return ad2(a, b); }
     static MM_HD B sb(B a, B b) { return sb2(a, b); }
     static MM_HD B rot(B a) { return rot2(a); }
+    // First-stage add/sub of a butterfly, unreduced: from lazy inputs (<= 2^61 + 6) the results
+    // stay below 3 * 2^61 + 6, which the folded second-stage ad2/sb2 (K4 = 4 P2) accept.
+    static MM_HD B adL(B a, B b) { return a + b; }
+    static MM_HD B sbL(B a, B b) { return a + (2 * P2 - b); }
     // x * w: re = x.re w.re + x.im (P2 - w.im), im = x.re w.im + x.im w.re.
     static MM_HD Z mul(Z x, C w) {
         Sp a = spl(x.re), b = spl(x.im);
@@ -169,6 +176,16 @@ template <class O> MM_HD typename O::Z zadd(typename O::Z a, typename O::Z b) {
 template <class O> MM_HD typename O::Z zsub(typename O::Z a, typename O::Z b) {
     return {O::sb(a.re, b.re), O::sb(a.im, b.im)};
 }
+// The same for a butterfly's first stage (unreduced in GF(M61)).
+template <class O> MM_HD typename O::Z zaddL(typename O::Z a, typename O::Z b) {
+    return {O::adL(a.re, b.re), O::adL(a.im, b.im)};
+}
+template <class O> MM_HD typename O::Z zsubL(typename O::Z a, typename O::Z b) {
+    return {O::sbL(a.re, b.re), O::sbL(a.im, b.im)};
+}
+template <class O> MM_HD typename O::Z zsub_iL(typename O::Z a, typename O::Z b) {
+    return {O::sbL(b.im, a.im), O::sbL(a.re, b.re)};
+}
 // (a - b) * i
 template <class O> MM_HD typename O::Z zsub_i(typename O::Z a, typename O::Z b) {
     return {O::sb(b.im, a.im), O::sb(a.re, b.re)};
@@ -182,12 +199,12 @@ template <class O> MM_HD typename O::Z zsub_mi(typename O::Z a, typename O::Z b)
 // DIF radix 8 on v[o], v[o+1], ..., v[o+7]: output slot m holds frequency rev3(m).
 template <class O> MM_HD void dif8(typename O::Z* v) {
     typedef typename O::Z Z;
-    Z a0 = zadd<O>(v[0], v[4]), a1 = zadd<O>(v[1], v[5]), a2 = zadd<O>(v[2], v[6]), a3 = zadd<O>(v[3], v[7]);
-    Z a4 = zsub<O>(v[0], v[4]);
-    Z d1 = zsub<O>(v[1], v[5]);
+    Z a0 = zaddL<O>(v[0], v[4]), a1 = zaddL<O>(v[1], v[5]), a2 = zaddL<O>(v[2], v[6]), a3 = zaddL<O>(v[3], v[7]);
+    Z a4 = zsubL<O>(v[0], v[4]);
+    Z d1 = zsubL<O>(v[1], v[5]);
     Z a5{O::rot(O::sb(d1.re, d1.im)), O::rot(O::ad(d1.re, d1.im))};  // d1 2^h (1 + i)
-    Z a6 = zsub_i<O>(v[2], v[6]);                                    // (v2 - v6) i
-    Z e3 = zsub<O>(v[7], v[3]);                                      // -(v3 - v7)
+    Z a6 = zsub_iL<O>(v[2], v[6]);                                   // (v2 - v6) i
+    Z e3 = zsubL<O>(v[7], v[3]);                                     // -(v3 - v7)
     Z a7{O::rot(O::ad(e3.re, e3.im)), O::rot(O::sb(e3.im, e3.re))};  // (v3 - v7) 2^h (-1 + i)
     Z b0 = zadd<O>(a0, a2), b2 = zsub<O>(a0, a2), b1 = zadd<O>(a1, a3), b3 = zsub_i<O>(a1, a3);
     Z b4 = zadd<O>(a4, a6), b6 = zsub<O>(a4, a6), b5 = zadd<O>(a5, a7), b7 = zsub_i<O>(a5, a7);
@@ -202,7 +219,7 @@ template <class O> MM_HD void dif8(typename O::Z* v) {
 }
 template <class O> MM_HD void dif4(typename O::Z* v) {
     typedef typename O::Z Z;
-    Z a0 = zadd<O>(v[0], v[2]), a2 = zsub<O>(v[0], v[2]), a1 = zadd<O>(v[1], v[3]), a3 = zsub_i<O>(v[1], v[3]);
+    Z a0 = zaddL<O>(v[0], v[2]), a2 = zsubL<O>(v[0], v[2]), a1 = zaddL<O>(v[1], v[3]), a3 = zsub_iL<O>(v[1], v[3]);
     v[0] = zadd<O>(a0, a1);
     v[1] = zsub<O>(a0, a1);
     v[2] = zadd<O>(a2, a3);
@@ -216,8 +233,8 @@ template <class O> MM_HD void dif2(typename O::Z* v) {
 // DIT radix 8 (inverse roots): input slot m holds frequency rev3(m), output natural.
 template <class O> MM_HD void dit8(typename O::Z* v) {
     typedef typename O::Z Z;
-    Z b0 = zadd<O>(v[0], v[1]), b1 = zsub<O>(v[0], v[1]), b2 = zadd<O>(v[2], v[3]), b3 = zsub<O>(v[2], v[3]);
-    Z b4 = zadd<O>(v[4], v[5]), b5 = zsub<O>(v[4], v[5]), b6 = zadd<O>(v[6], v[7]), b7 = zsub<O>(v[6], v[7]);
+    Z b0 = zaddL<O>(v[0], v[1]), b1 = zsubL<O>(v[0], v[1]), b2 = zaddL<O>(v[2], v[3]), b3 = zsubL<O>(v[2], v[3]);
+    Z b4 = zaddL<O>(v[4], v[5]), b5 = zsubL<O>(v[4], v[5]), b6 = zaddL<O>(v[6], v[7]), b7 = zsubL<O>(v[6], v[7]);
     // span 2: second element of pairs (1,3), (5,7) times -i.
     Z c0 = zadd<O>(b0, b2), c2 = zsub<O>(b0, b2);
     Z c1{O::ad(b1.re, b3.im), O::sb(b1.im, b3.re)}, c3{O::sb(b1.re, b3.im), O::ad(b1.im, b3.re)};
@@ -237,7 +254,7 @@ template <class O> MM_HD void dit8(typename O::Z* v) {
 }
 template <class O> MM_HD void dit4(typename O::Z* v) {
     typedef typename O::Z Z;
-    Z b0 = zadd<O>(v[0], v[1]), b1 = zsub<O>(v[0], v[1]), b2 = zadd<O>(v[2], v[3]), b3 = zsub<O>(v[2], v[3]);
+    Z b0 = zaddL<O>(v[0], v[1]), b1 = zsubL<O>(v[0], v[1]), b2 = zaddL<O>(v[2], v[3]), b3 = zsubL<O>(v[2], v[3]);
     v[0] = zadd<O>(b0, b2);
     v[2] = zsub<O>(b0, b2);
     v[1] = Z{O::ad(b1.re, b3.im), O::sb(b1.im, b3.re)};

@@ -276,3 +276,83 @@ def carry_stress(make, max_len):
             want = pow(x, 1 << 32, m) * pow(3, e, m) % m
             assert b.read(i) == want, f"len {length} slot {i}"
         length *= 2
+
+
+# ---- Transform layer (common/mod.rs:23-91) -------------------------------------------------
+# `t` is an adapter over a transform backend with upload(coeffs, len, count) -> buf,
+# forward/inverse(buf, len, count), pointwise(out, a, b, scales, len, count) and
+# download_words(buf) -> sequence of 64-bit words, two (low, high) per coefficient.
+
+
+def coeffs_to_int(words):
+    """sum_q c_q 2^(32 q) for coefficients given as (low, high) word pairs, by a linear carry."""
+    limbs, carry = [], 0
+    for q in range(len(words) // 2):
+        carry += int(words[2 * q]) | (int(words[2 * q + 1]) << 64)
+        limbs.append(carry & 0xFFFFFFFF)
+        carry >>= 32
+    while carry:
+        limbs.append(carry & 0xFFFFFFFF)
+        carry >>= 32
+    return int.from_bytes(b"".join(l.to_bytes(4, "little") for l in limbs), "little")
+
+
+def exactness(t, max_len):
+    """All-ones limbs at k = L/2 times themselves with scale 4: coefficient q equals
+    4 (2^32 - 1)^2 min(q + 1, 2k - 1 - q) and the whole product is (R - 1)^2 4 (common/mod.rs:23-53)."""
+    import numpy as np
+
+    e = 3
+    while (1 << e) <= max_len:
+        length = 1 << e
+        k, c = length // 2, 4
+        coeffs = np.zeros(length, dtype=np.uint32)
+        coeffs[:k] = 0xFFFFFFFF
+        x, y = t.upload(coeffs, length, 1), t.upload(coeffs, length, 1)
+        t.forward(x, length, 1)
+        t.forward(y, length, 1)
+        out = t.upload(np.zeros(length, dtype=np.uint32), length, 1)
+        t.pointwise(out, x, y, [c], length, 1)
+        t.inverse(out, length, 1)
+        w = np.asarray(t.download_words(out), dtype=np.uint64).reshape(-1, 2)
+        sq = c * (2**32 - 1) ** 2
+        q = np.arange(length, dtype=np.int64)
+        cnt = np.where(q < k, q + 1, np.maximum(2 * k - 1 - q, 0))
+        want = [sq * int(v) for v in cnt]
+        got = [int(lo) | (int(hi) << 64) for lo, hi in w]
+        assert got == want, f"len {length}"
+        a = (1 << (32 * k)) - 1
+        assert coeffs_to_int(w.reshape(-1)) == a * a * c, f"len {length}"
+        e += 1
+
+
+def transform_batched(t, max_len, seed=7):
+    """Three vectors per call, products checked against integer multiplication, with scales
+    [1, 3, 4] and with none (common/mod.rs:55-91)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    e = 3
+    while (1 << e) <= max_len:
+        length, count = 1 << e, 3
+        k = length // 2
+        cs = []
+        for _ in range(2):
+            c = np.zeros(length * count, dtype=np.uint32)
+            for v in range(count):
+                c[v * length:v * length + k] = rng.integers(0, 2**32, size=k, dtype=np.uint64).astype(np.uint32)
+            cs.append(c)
+        for scales in ([1, 3, 4], None):
+            x, y = t.upload(cs[0], length, count), t.upload(cs[1], length, count)
+            t.forward(x, length, count)
+            t.forward(y, length, count)
+            out = t.upload(np.zeros(length * count, dtype=np.uint32), length, count)
+            t.pointwise(out, x, y, scales, length, count)
+            t.inverse(out, length, count)
+            w = np.asarray(t.download_words(out), dtype=np.uint64)
+            for v in range(count):
+                val = coeffs_to_int(w[2 * v * length:2 * (v + 1) * length])
+                a = int.from_bytes(cs[0][v * length:(v + 1) * length].tobytes(), "little")
+                b = int.from_bytes(cs[1][v * length:(v + 1) * length].tobytes(), "little")
+                assert val == a * b * (1 if scales is None else scales[v]), f"len {length} vector {v}"
+        e += 1

@@ -12,3 +12,23 @@ for p in (ROOT, os.path.join(ROOT, "tests")):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     config.addinivalue_line("markers", "slow: long-running case")
+
+
+@pytest.fixture(scope="module")
+def hip():
+    """The HIP backend on device 0 (LDS or global mode as planned)."""
+    from montmul_amd import Hip
+
+    h = Hip()
+    yield h
+    h.close()
+
+
+@pytest.fixture(scope="module")
+def hip_global():
+    """The HIP backend with global mode forced (GpuOptions::force_global)."""
+    from montmul_amd import Hip
+
+    h = Hip(force_global=True)
+    yield h
+    h.close()

@@ -159,3 +159,30 @@ def gmp_step(x, n, nprime, k, c):
     out = np.zeros(k, dtype=np.uint32)
     olib.og_step(_p32(xs), _p32(ns), _p32(ps), k, c, _p32(out))
     return int.from_bytes(out.tobytes(), "little")
+
+
+class CpuTransform:
+    """The oracle's transform layer (Cpu's Transform impl, cpu.rs:349-407) behind the adapter
+    interface of common.exactness / common.transform_batched."""
+
+    def upload(self, coeffs, length, count):
+        return t_upload(coeffs, length, count)
+
+    def forward(self, b, length, count):
+        assert (b.len, b.count) == (length, count)
+        t_forward(b)
+
+    def inverse(self, b, length, count):
+        assert (b.len, b.count) == (length, count)
+        t_inverse(b)
+
+    def pointwise(self, out, a, b, scales, length, count):
+        t_pointwise(out, a, b, scales)
+
+    def download_words(self, b):
+        out = np.zeros(2 * b.len * b.count, dtype=np.uint64)
+        olib.oc_t_download(_p32(b.f1), _p64(b.f2), b.len, b.count, _p64(out))
+        return out
+
+    def download_raw(self, b):
+        return b.f1, b.f2

@@ -24,20 +24,6 @@ def say(*a):
     print(*a, flush=True)
 
 
-@pytest.fixture(scope="module")
-def hip():
-    h = Hip()
-    yield h
-    h.close()
-
-
-@pytest.fixture(scope="module")
-def hip_global():
-    h = Hip(force_global=True)
-    yield h
-    h.close()
-
-
 def test_device(hip):
     say("device", hip.device_name(), "CUs", hip.cu_count(), "LDS", hip.max_lds())
     assert Hip.device_count() >= 1
@@ -272,3 +258,28 @@ def test_headline_shape_closed_form(hip, hip_global):
         bad = [i for i in range(256) if b.read_raw(i) != ref.read_raw(i)]
         assert not bad, f"{name}: slots {bad[:8]} differ"
         b.free()
+
+
+def test_cpu_case_4096_bits(hip):
+    """The reference's CPU-runnable case: 64 moduli of 4,096 bits (L = 512, k = 256) from the
+    score workload seeded 2026, e = (N - 1) >> 3,584; arm, run(16), re-arm, run(512). Every slot:
+    read == 3^e mod N and raw == the closed-form chain (GMP, k = L/2)."""
+    moduli, exps = mm.score_workload(4096, 64, 512, 2026)
+    assert all(e == (n - 1) >> 3584 for n, e in zip(moduli, exps))
+    b = ModBatch(hip, moduli, 3)
+    assert (b.params().k, b.params().len) == (256, 512)
+    ref = ModBatch(Gmp(full_k=True), moduli, 3)
+    for bb in (b, ref):
+        for i, e in enumerate(exps):
+            bb.load(i, 1)
+            bb.set_exponent(i, e)
+    b.run(16)
+    for i, e in enumerate(exps):
+        b.load(i, 1)
+        b.set_exponent(i, e)
+    b.run(512)
+    ref.run(512)
+    for i, (n, e) in enumerate(zip(moduli, exps)):
+        assert b.remaining(i) == 0
+        assert b.read(i) == pow(3, e, n), f"slot {i}"
+        assert b.read_raw(i) == ref.read_raw(i), f"slot {i} raw"

@@ -16,7 +16,7 @@ import pytest
 
 import common
 from montmul_amd import ModBatch, fermat
-from oracle_backends import Cpu, Gmp, gmp_step, olib, t_download, t_forward, t_inverse, t_pointwise, t_upload
+from oracle_backends import Cpu, CpuTransform, Gmp, gmp_step, olib, t_download, t_forward, t_inverse, t_pointwise, t_upload
 import ctypes as C
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -112,51 +112,12 @@ def test_dif_is_brute_force_dft_in_bit_reversed_order(n):
 
 def test_exactness_closed_form():
     """All-ones limbs at k = L/2, c = 4 (common/mod.rs:23-53)."""
-    for e in range(3, 15):
-        length = 1 << e
-        k, c = length // 2, 4
-        coeffs = np.zeros(length, dtype=np.uint32)
-        coeffs[:k] = 0xFFFFFFFF
-        x, y = t_upload(coeffs, length, 1), t_upload(coeffs, length, 1)
-        t_forward(x)
-        t_forward(y)
-        out = t_upload(np.zeros(length, dtype=np.uint32), length, 1)
-        t_pointwise(out, x, y, [c])
-        t_inverse(out)
-        got = t_download(out)
-        sq = 2**32 - 1
-        for q in range(length):
-            cnt = q + 1 if q < k else (2 * k - 1 - q if q < 2 * k - 1 else 0)
-            assert got[q] == c * sq * sq * cnt, f"len {length} coeff {q}"
-        a = (1 << (32 * k)) - 1
-        assert sum(v << (32 * q) for q, v in enumerate(got)) == a * a * c
+    common.exactness(CpuTransform(), 1 << 14)
 
 
 def test_transform_batched_products():
     """Three vectors per call with scales [1, 3, 4] and none (common/mod.rs:55-91)."""
-    rng = random.Random(7)
-    for e in range(3, 12):
-        length, count = 1 << e, 3
-        k = length // 2
-        cs = []
-        for _ in range(2):
-            c = np.zeros(length * count, dtype=np.uint32)
-            for v in range(count):
-                c[v * length:v * length + k] = [rng.getrandbits(32) for _ in range(k)]
-            cs.append(c)
-        for scales in ([1, 3, 4], None):
-            x, y = t_upload(cs[0], length, count), t_upload(cs[1], length, count)
-            t_forward(x)
-            t_forward(y)
-            out = t_upload(np.zeros(length * count, dtype=np.uint32), length, count)
-            t_pointwise(out, x, y, scales)
-            t_inverse(out)
-            got = t_download(out)
-            for v in range(count):
-                val = sum(c << (32 * q) for q, c in enumerate(got[v * length:(v + 1) * length]))
-                a = int.from_bytes(cs[0][v * length:(v + 1) * length].tobytes(), "little")
-                b = int.from_bytes(cs[1][v * length:(v + 1) * length].tobytes(), "little")
-                assert val == a * b * (1 if scales is None else scales[v])
+    common.transform_batched(CpuTransform(), 1 << 11)
 
 
 @pytest.mark.parametrize("h", [2, 4, 8, 256])
@@ -260,3 +221,21 @@ def test_oracle_flags_carry_out():
     with pytest.raises(CarryError) as ei:
         b.run(1)
     assert ei.value.slot == 1 and "slot 1" in str(ei.value)
+
+
+def test_cpu_case_4096_bits_oracle():
+    """The reference's CPU-runnable case on the oracle's Cpu restatement (wraparound, k = L/2):
+    64 moduli of 4,096 bits from the score workload seeded 2026, e = (N - 1) >> 3,584; every slot's
+    read equals 3^e mod N and its raw limbs the GMP closed-form chain."""
+    import montmul_amd as mm
+
+    moduli, exps = mm.score_workload(4096, 64, 512, 2026)
+    a, b = ModBatch(Cpu(wraparound=True), moduli, 3), ModBatch(Gmp(full_k=True), moduli, 3)
+    for bb in (a, b):
+        for i, e in enumerate(exps):
+            bb.load(i, 1)
+            bb.set_exponent(i, e)
+        bb.run(512)
+    for i, (n, e) in enumerate(zip(moduli, exps)):
+        assert a.read(i) == pow(3, e, n), f"slot {i}"
+        assert a.read_raw(i) == b.read_raw(i), f"slot {i} raw"
