@@ -31,6 +31,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "squarings/s at 100k-bit, 1 MI355X"
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+VALU_PEAK = 256 * 2.4e9    # wave64 VALU instructions/s: one per cycle per CU, 256 CUs at 2.4 GHz
+PROFILE = "profiles/k_steps_v2_global_100kbit_4096slots.md"
+# Per slot-step of the global-mode step kernel at L = 8,192, from PROFILE: HBM bytes moved
+# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, scales from a 1 GiB device copy in the same run) and
+# VALU wave instructions (SQ_INSTS_VALU).
+TRAFFIC_PER_SLOT_STEP = {8192: 506_735}
+VALU_PER_SLOT_STEP = {8192: 172_143}
 REF_5090 = 1_536_441.0     # reference, RTX 5090, 170 slots (README.md:14)
 
 
@@ -125,6 +132,27 @@ def cpu_baseline(bits, ebits, threads):
     return out
 
 
+def copy_calibration(nbytes):
+    """Four device-to-device copies of `nbytes` (the HIP runtime's blit kernel,
+    __amd_rocclr_copyBuffer): a plain copy of known size whose FETCH_SIZE / WRITE_SIZE counters
+    calibrate the step kernel's in a profiled run."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise SystemExit("copy calibration: hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    for _ in range(4):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise SystemExit("copy calibration: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "launches": 4, "kernel": "__amd_rocclr_copyBuffer"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,10 +233,19 @@ def main():
         "mad": round(mad(head["rates"]), 1),
         "fermat_tests_per_hour": round(value * 3600 / args.bits, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
-                     "kernel": "k_steps (global mode)", "kernel_ms_per_launch": round(kms, 3),
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
+                     "traffic": (TRAFFIC_PER_SLOT_STEP[L] * slots * ebits) if L in TRAFFIC_PER_SLOT_STEP else None,
+                     "traffic_per_slot_step": TRAFFIC_PER_SLOT_STEP.get(L), "traffic_source": PROFILE,
+                     "kernel": "k_steps_v2 (global mode)", "kernel_ms_per_launch": round(kms, 3),
                      "alg_bytes_per_slot_step": 28 * L},
         "checked_slots": head["checked"],
+        # The bound that binds: VALU issue (integer work, no MFMA). Instructions per slot-step
+        # from the profile, rate from this run's kernel time.
+        "valu": ({"bound": "valu", "achieved": round(VALU_PER_SLOT_STEP[L] * slots * ebits / (kms * 1e-3) / 1e9, 2),
+                  "peak": VALU_PEAK / 1e9, "unit": "G wave-instr/s",
+                  "frac": round(VALU_PER_SLOT_STEP[L] * slots * ebits / (kms * 1e-3) / VALU_PEAK, 4),
+                  "instr_per_slot_step": VALU_PER_SLOT_STEP[L], "source": PROFILE}
+                 if L in VALU_PER_SLOT_STEP else None),
         "sweep": head["sweep"],
     }
     if "lds" in results and "global" in results:
@@ -226,15 +263,7 @@ def main():
                                 "cpu_ntt_value": round(cb["cpu_ntt"], 1),
                                 "cpu_ntt_sample": "oracle Cpu restatement (cpu.rs NTT, wraparound): " + cb["cpu_sample"]}
     if args.copy_calibration:
-        import torch
-
-        nbytes = args.copy_calibration << 20
-        src = torch.empty(nbytes // 4, dtype=torch.int32, device="cuda").fill_(1)
-        dst = torch.empty_like(src)
-        for _ in range(4):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        line["copy_calibration"] = {"bytes_read": nbytes, "bytes_written": nbytes, "launches": 4}
+        line["copy_calibration"] = copy_calibration(args.copy_calibration << 20)
     print(json.dumps(line), flush=True)
 
 

@@ -602,10 +602,14 @@ struct State {
     uint32_t ew = 0;
     // host mirrors
     std::vector<uint32_t> h_x;
-    bool x_host_valid = true, x_dirty = false;
+    bool x_host_valid = true;         // every row of h_x equals the device's x
+    std::vector<uint8_t> x_dirty;     // rows written on the host since the last upload
+    std::vector<uint32_t> x_dirty_list;
     std::vector<std::vector<uint32_t>> exps;
     std::vector<uint32_t> nbits, pos;
-    bool exp_dirty = false;
+    std::vector<uint8_t> e_dirty;     // exponents set on the host since the last upload
+    std::vector<uint32_t> e_dirty_list;
+    bool exp_all = true;              // the device exponent buffer needs a full upload
     double ms_per_slot_step[2] = {0, 0};
     double last_ms = 0;
     uint32_t last_launches = 0;
@@ -708,35 +712,74 @@ int set_lds_limit(Ctx* C) {
     return 0;
 }
 
+// Uploads the rows of x and the exponents written on the host since the last upload: runs of
+// consecutive dirty slots go in one copy each, so that admitting a few slots into a large batch
+// moves only their rows (reference: Gpu::push, gpu/mod.rs:161-176, which uploads everything).
 int push(State* S) {
-    if (S->x_dirty) {
-        HIPCHK(hipMemcpyAsync(S->d_x, S->h_x.data(), S->h_x.size() * 4, hipMemcpyHostToDevice, S->C->stream));
-        S->x_dirty = false;
+    hipStream_t sm = S->C->stream;
+    auto runs = [&](std::vector<uint32_t>& list, auto&& copy) -> int {
+        std::sort(list.begin(), list.end());
+        for (size_t a = 0; a < list.size();) {
+            size_t b = a + 1;
+            while (b < list.size() && list[b] == list[b - 1] + 1) b++;
+            if (copy(list[a], list[b - 1] + 1 - list[a])) return 1;
+            a = b;
+        }
+        return 0;
+    };
+    if (!S->x_dirty_list.empty()) {
+        const uint32_t k = S->k;
+        if (runs(S->x_dirty_list, [&](uint32_t first, uint32_t cnt) -> int {
+                HIPCHK(hipMemcpyAsync(S->d_x + (size_t)first * k, S->h_x.data() + (size_t)first * k, (size_t)cnt * k * 4,
+                                      hipMemcpyHostToDevice, sm));
+                return 0;
+            }))
+            return 1;
+        for (uint32_t i : S->x_dirty_list) S->x_dirty[i] = 0;
+        S->x_dirty_list.clear();
     }
-    if (S->exp_dirty) {
-        uint32_t ew = 1;
-        for (auto& e : S->exps) ew = std::max<uint32_t>(ew, (uint32_t)e.size());
+    if (!S->e_dirty_list.empty() || S->exp_all) {
+        uint32_t ew = S->ew;
+        for (uint32_t i : S->e_dirty_list) ew = std::max<uint32_t>(ew, (uint32_t)S->exps[i].size());
         if (ew > S->ew || !S->d_exp) {
-            HIPCHK(hipStreamSynchronize(S->C->stream));
+            // Longer exponents: a wider buffer, every row uploaded again.
+            HIPCHK(hipStreamSynchronize(sm));
             if (S->d_exp) hipFree(S->d_exp);
             S->d_exp = nullptr;
             if (dalloc(&S->d_exp, (size_t)S->slots * ew)) return 1;
             S->ew = ew;
+            S->exp_all = true;
         }
-        std::vector<uint32_t> words((size_t)S->slots * S->ew, 0);
-        for (uint32_t i = 0; i < S->slots; i++)
-            std::copy(S->exps[i].begin(), S->exps[i].end(), words.begin() + (size_t)i * S->ew);
-        HIPCHK(hipMemcpyAsync(S->d_exp, words.data(), words.size() * 4, hipMemcpyHostToDevice, S->C->stream));
-        HIPCHK(hipMemcpyAsync(S->d_nbits, S->nbits.data(), S->slots * 4, hipMemcpyHostToDevice, S->C->stream));
-        HIPCHK(hipMemcpyAsync(S->d_pos, S->pos.data(), S->slots * 4, hipMemcpyHostToDevice, S->C->stream));
-        HIPCHK(hipStreamSynchronize(S->C->stream));
-        S->exp_dirty = false;
+        std::vector<uint32_t> all;
+        if (S->exp_all) {
+            all.resize(S->slots);
+            for (uint32_t i = 0; i < S->slots; i++) all[i] = i;
+        }
+        std::vector<uint32_t>& list = S->exp_all ? all : S->e_dirty_list;
+        std::vector<uint32_t> words;
+        if (runs(list, [&](uint32_t first, uint32_t cnt) -> int {
+                words.assign((size_t)cnt * S->ew, 0);
+                for (uint32_t i = 0; i < cnt; i++)
+                    std::copy(S->exps[first + i].begin(), S->exps[first + i].end(), words.begin() + (size_t)i * S->ew);
+                HIPCHK(hipMemcpyAsync(S->d_exp + (size_t)first * S->ew, words.data(), words.size() * 4, hipMemcpyHostToDevice, sm));
+                // the staging vector is reused: finish this copy first
+                HIPCHK(hipStreamSynchronize(sm));
+                return 0;
+            }))
+            return 1;
+        HIPCHK(hipMemcpyAsync(S->d_nbits, S->nbits.data(), S->slots * 4, hipMemcpyHostToDevice, sm));
+        HIPCHK(hipMemcpyAsync(S->d_pos, S->pos.data(), S->slots * 4, hipMemcpyHostToDevice, sm));
+        HIPCHK(hipStreamSynchronize(sm));
+        for (uint32_t i : S->e_dirty_list) S->e_dirty[i] = 0;
+        S->e_dirty_list.clear();
+        S->exp_all = false;
     }
     return 0;
 }
 
 int pull_x(State* S) {
     if (S->x_host_valid) return 0;
+    if (push(S)) return 1;  // rows written on the host go to the device first
     HIPCHK(hipMemcpyAsync(S->h_x.data(), S->d_x, S->h_x.size() * 4, hipMemcpyDeviceToHost, S->C->stream));
     HIPCHK(hipStreamSynchronize(S->C->stream));
     S->x_host_valid = true;
@@ -972,7 +1015,9 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         HIPCHK(hipStreamSynchronize(sm));
     }
     S->h_x.assign(sl * k, 0);
+    S->x_dirty.assign(sl, 0);
     S->exps.assign(sl, {});
+    S->e_dirty.assign(sl, 0);
     S->nbits.assign(sl, 0);
     S->pos.assign(sl, 0);
     *out = S.release();
@@ -984,9 +1029,11 @@ void mm_destroy(void* st) { delete (State*)st; }
 int mm_load(void* st, size_t i, const uint32_t* limbs) {  // gpu/mod.rs:371-375
     State* S = (State*)st;
     if (i >= S->slots) return fail("slot out of range");
-    if (hipSetDevice(S->C->device) != hipSuccess || pull_x(S)) return fail("load: cannot read device values");
     std::copy(limbs, limbs + S->k, S->h_x.begin() + i * S->k);
-    S->x_dirty = true;
+    if (!S->x_dirty[i]) {
+        S->x_dirty[i] = 1;
+        S->x_dirty_list.push_back((uint32_t)i);
+    }
     return 0;
 }
 
@@ -997,7 +1044,10 @@ int mm_set_exponent(void* st, size_t i, const uint32_t* words, size_t nwords, ui
     S->exps[i].assign(words, words + nwords);
     S->nbits[i] = (uint32_t)nbits;
     S->pos[i] = 0;
-    S->exp_dirty = true;
+    if (!S->e_dirty[i]) {
+        S->e_dirty[i] = 1;
+        S->e_dirty_list.push_back((uint32_t)i);
+    }
     return 0;
 }
 
@@ -1032,8 +1082,14 @@ int64_t mm_mul(void* st, size_t out, size_t a, size_t b) {  // gpu/mod.rs:386-39
 int mm_read(void* st, size_t i, uint32_t* out) {  // gpu/mod.rs:392-394
     State* S = (State*)st;
     if (i >= S->slots) return fail("slot out of range");
-    if (hipSetDevice(S->C->device) != hipSuccess || pull_x(S)) return 1;
-    std::copy(S->h_x.begin() + i * S->k, S->h_x.begin() + (i + 1) * S->k, out);
+    if (S->x_host_valid || S->x_dirty[i]) {
+        std::copy(S->h_x.begin() + i * S->k, S->h_x.begin() + (i + 1) * S->k, out);
+        return 0;
+    }
+    // The device row alone (the reference downloads every slot here, gpu/mod.rs:247-269).
+    if (hipSetDevice(S->C->device) != hipSuccess) return fail("hipSetDevice failed");
+    HIPCHK(hipMemcpyAsync(out, S->d_x + i * S->k, (size_t)S->k * 4, hipMemcpyDeviceToHost, S->C->stream));
+    HIPCHK(hipStreamSynchronize(S->C->stream));
     return 0;
 }
 
