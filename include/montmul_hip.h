@@ -68,6 +68,11 @@ typedef struct mm_backend {
     int (*read)(void *state, size_t slot, uint32_t *out);
     /* Backend::remaining (transform.rs:76). */
     uint64_t (*remaining)(void *state, size_t slot);
+    /* Reprograms slot `slot` for another modulus of the batch's k and L (N, N' and 2N as k
+     * limbs each, as in mm_batch_setup) and clears its exponent; x is left as is. NULL when the
+     * backend cannot. No reference counterpart: it admits new work into a running batch
+     * (serving) without rebuilding the batch. */
+    int (*set_modulus)(void *state, size_t slot, const uint32_t *n, const uint32_t *nprime, const uint32_t *two_n);
     /* Message of the calling thread's last error of this backend. */
     const char *(*last_error)(void);
     /* Limits::coefficient_bound (transform.rs:13), as a 128-bit value. */
@@ -129,6 +134,8 @@ uint64_t mm_free_memory(void *self);
 int mm_cu_count(void *self);
 uint64_t mm_max_lds(void *self);
 int mm_device_name(void *self, char *buf, size_t len);
+/* The Hip table's set_modulus (see mm_backend): device constants of one slot rebuilt. */
+int mm_set_modulus(void *state, size_t slot, const uint32_t *n, const uint32_t *nprime, const uint32_t *two_n);
 /* Device time of the step kernel launches of the latest run, in milliseconds (HIP events on
  * the batch's stream), and the number of launches. No reference counterpart: measurement. */
 int mm_last_run_kernel_ms(void *state, double *ms, uint32_t *launches);
@@ -214,6 +221,11 @@ int mm_modbatch_read(mm_modbatch *b, size_t slot, uint32_t *out);
 int mm_modbatch_is_one(mm_modbatch *b, size_t slot);
 /* k limbs of slot i's modulus (ModBatch::modulus, batch.rs:113-115). */
 int mm_modbatch_modulus(mm_modbatch *b, size_t slot, uint32_t *out);
+/* Replaces slot i's modulus by another odd N > 1 needing the batch's L (k limbs at most): N', R
+ * mod N, R^-1 mod N and 2N are computed as in mm_modbatch_new and the backend's set_modulus
+ * rebuilds the slot; its exponent is cleared. Fails when the backend has no set_modulus. No
+ * reference counterpart (the reference rebuilds a ModBatch): admission for serving. */
+int mm_modbatch_set_modulus(mm_modbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
 
 /* fermat (batch.rs:187-198): base-`base` Fermat PRP test of every modulus; prp[i] = 1 or 0. */
 int mm_fermat(const mm_backend *be, size_t count, const uint32_t *limbs, const size_t *offsets,
@@ -231,6 +243,57 @@ int mm_score_workload(uint32_t bits, size_t slots, uint32_t ebits, uint64_t seed
 
 /* Message of the calling thread's last host-layer error. */
 const char *mm_host_last_error(void);
+
+/* ------------------------------------------------------------------------------------ */
+/* Serving: base-c Fermat PRP requests over length pools (no reference counterpart; the  */
+/* reference runs fixed batches). Each transform length gets a pool, a batch of           */
+/* `pool_slots` slots on a backend table of its own driven by one host thread: a round     */
+/* admits waiting requests into free slots (mm_modbatch_set_modulus), runs a chunk of      */
+/* steps sized to `target_ms`, and retires finished slots (is_one on that slot's raw).     */
+/* Pools run concurrently (on the HIP backend: one stream each).                           */
+/* ------------------------------------------------------------------------------------ */
+
+typedef struct mm_server mm_server;
+/* Fills `out` with a fresh backend table (each pool opens its own) / releases it. */
+typedef int (*mm_open_fn)(void *user, mm_backend *out);
+typedef void (*mm_close_fn)(void *user, mm_backend *be);
+
+typedef struct mm_server_options {
+    uint32_t base;        /* Fermat base c, 1 .. 4 */
+    uint32_t pool_slots;  /* slots per length pool (0: 256) */
+    double target_ms;     /* wall time of one pool round; steps per round follow from it */
+    uint32_t max_chunk;   /* most steps per round (0: 4096) */
+    int32_t keep_raw;     /* keep each result's final raw limbs for mm_server_raw */
+} mm_server_options;
+
+typedef struct mm_server_result {
+    int64_t id;
+    int32_t prp;          /* 1: c^(N-1) = 1 mod N; 0: not; -1: failed */
+    uint32_t len;         /* transform length of its pool */
+    uint64_t bits;        /* bits of N - 1 (the squarings it took) */
+    double submit_s, admit_s, done_s;  /* seconds since the server started */
+} mm_server_result;
+
+typedef struct mm_server_stats {
+    uint64_t submitted, admitted, completed, failed, rounds, squarings, waiting;
+    double busy_ms;       /* wall time of the pools' runs, summed over pools */
+    uint32_t pools;
+} mm_server_stats;
+
+int mm_server_new(const mm_server_options *opts, mm_open_fn open, mm_close_fn close, void *user, mm_server **out);
+/* A server whose pools are HIP backends opened with `hip` (one context and stream per pool). */
+int mm_server_new_hip(const mm_server_options *opts, const mm_options *hip, mm_server **out);
+/* Queues a Fermat test of the odd N > 1 given as little-endian limbs. Returns its id (>= 0),
+ * or -1 with mm_server_last_error(). */
+int64_t mm_server_submit(mm_server *s, const uint32_t *limbs, size_t nlimbs);
+/* Up to `max` completions, waiting up to `wait_s` seconds for the first. Returns the count. */
+size_t mm_server_poll(mm_server *s, mm_server_result *out, size_t max, double wait_s);
+/* The final raw Montgomery limbs of a completed request (when keep_raw), once. */
+int mm_server_raw(mm_server *s, int64_t id, uint32_t *out, size_t max_limbs, size_t *nlimbs);
+int mm_server_stats_get(mm_server *s, mm_server_stats *out);
+const char *mm_server_last_error(mm_server *s);
+/* Stops every pool (requests in flight are dropped) and frees the server. */
+void mm_server_free(mm_server *s);
 
 #ifdef __cplusplus
 }

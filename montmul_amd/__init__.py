@@ -56,6 +56,8 @@ _RUN = C.CFUNCTYPE(C.c_int64, C.c_void_p, C.c_uint64)
 _MUL = C.CFUNCTYPE(C.c_int64, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t)
 _READ = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32))
 _REMAINING = C.CFUNCTYPE(C.c_uint64, C.c_void_p, C.c_size_t)
+_SETMOD = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                      C.POINTER(C.c_uint32))
 _LASTERR = C.CFUNCTYPE(C.c_char_p)
 
 
@@ -63,14 +65,36 @@ class BackendTable(C.Structure):
     """`mm_backend`: the backend as a table of functions (transform.rs:11-20, 61-77)."""
     _fields_ = [("self", C.c_void_p), ("create", _CREATE), ("destroy", _DESTROY), ("load", _LOAD),
                 ("set_exponent", _SETEXP), ("run", _RUN), ("mul", _MUL), ("read", _READ),
-                ("remaining", _REMAINING), ("last_error", _LASTERR),
+                ("remaining", _REMAINING), ("set_modulus", _SETMOD), ("last_error", _LASTERR),
                 ("bound_lo", C.c_uint64), ("bound_hi", C.c_uint64),
                 ("min_len", C.c_uint32), ("max_len", C.c_uint32), ("full_k", C.c_int32)]
+
+
+_OPEN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(BackendTable))
+_CLOSE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(BackendTable))
 
 
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("force_global", C.c_int32), ("wraparound", C.c_int32),
                 ("max_launch_ms", C.c_uint32)]
+
+
+class ServerOptions(C.Structure):
+    """`mm_server_options`."""
+    _fields_ = [("base", C.c_uint32), ("pool_slots", C.c_uint32), ("target_ms", C.c_double),
+                ("max_chunk", C.c_uint32), ("keep_raw", C.c_int32)]
+
+
+class ServerResult(C.Structure):
+    """`mm_server_result`: one completed request."""
+    _fields_ = [("id", C.c_int64), ("prp", C.c_int32), ("len", C.c_uint32), ("bits", C.c_uint64),
+                ("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+                ("failed", C.c_uint64), ("rounds", C.c_uint64), ("squarings", C.c_uint64),
+                ("waiting", C.c_uint64), ("busy_ms", C.c_double), ("pools", C.c_uint32)]
 
 
 class PlanInfo(C.Structure):
@@ -115,6 +139,16 @@ def _load_lib():
         "mm_modbatch_read": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_modbatch_is_one": (C.c_int, [vp, C.c_size_t]),
         "mm_modbatch_modulus": (C.c_int, [vp, C.c_size_t, u32p]),
+        "mm_modbatch_set_modulus": (C.c_int, [vp, C.c_size_t, C.c_int, u32p, C.c_size_t]),
+        "mm_set_modulus": (C.c_int, [vp, C.c_size_t, u32p, u32p, u32p]),
+        "mm_server_new": (C.c_int, [C.POINTER(ServerOptions), _OPEN, _CLOSE, vp, C.POINTER(vp)]),
+        "mm_server_new_hip": (C.c_int, [C.POINTER(ServerOptions), C.POINTER(Options), C.POINTER(vp)]),
+        "mm_server_submit": (C.c_int64, [vp, u32p, C.c_size_t]),
+        "mm_server_poll": (C.c_size_t, [vp, C.POINTER(ServerResult), C.c_size_t, C.c_double]),
+        "mm_server_raw": (C.c_int, [vp, C.c_int64, u32p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "mm_server_stats_get": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "mm_server_last_error": (C.c_char_p, [vp]),
+        "mm_server_free": (None, [vp]),
         "mm_fermat": (C.c_int, [C.POINTER(BackendTable), C.c_size_t, u32p, C.POINTER(C.c_size_t),
                                 C.POINTER(C.c_int8), C.c_uint32, C.c_uint64, C.POINTER(C.c_uint8)]),
         "mm_score_workload": (C.c_int, [C.c_uint32, C.c_size_t, C.c_uint32, C.c_uint64, u32p, u32p]),
@@ -469,6 +503,15 @@ class ModBatch:
     def modulus(self, i):
         return self._moduli[i]
 
+    def set_modulus(self, i, n):
+        """Replaces slot i's modulus by another needing this batch's L; clears its exponent
+        (admission into a running batch; no reference counterpart)."""
+        n = int(n)
+        buf = to_limbs(abs(n))
+        if lib.mm_modbatch_set_modulus(self._h, i, _sign(n), buf, len(buf)) != 0:
+            raise MontmulError(_host_error())
+        self._moduli[i] = n
+
     def load(self, i, x):
         x = int(x)
         buf = to_limbs(x)
@@ -544,3 +587,74 @@ def score_workload(bits, slots, ebits, seed):
     moduli = [int.from_bytes(mb[4 * nl * i:4 * nl * (i + 1)], "little") for i in range(slots)]
     exps = [int.from_bytes(eb[4 * el * i:4 * el * (i + 1)], "little") for i in range(slots)]
     return moduli, exps
+
+
+class Server:
+    """Base-c Fermat PRP requests served over length pools (mm_server; no reference
+    counterpart). By default every pool is a HIP backend on `device`; `backend_factory` (a
+    callable returning a backend such as the oracle's) replaces it, one backend per pool."""
+
+    def __init__(self, base=3, pool_slots=256, target_ms=20.0, max_chunk=0, keep_raw=False, device=0,
+                 force_global=False, backend_factory=None):
+        self.options = ServerOptions(base, pool_slots, target_ms, max_chunk, int(bool(keep_raw)))
+        self._h = C.c_void_p()
+        self._keep = []
+        if backend_factory is None:
+            hip = Options(device, int(bool(force_global)), 1, 0)
+            r = lib.mm_server_new_hip(C.byref(self.options), C.byref(hip), C.byref(self._h))
+        else:
+            def _open(user, out):
+                try:
+                    be = backend_factory()
+                except Exception:
+                    return 1
+                self._keep.append(be)
+                out[0] = be.table
+                return 0
+
+            def _close(user, be):
+                return None
+
+            self._cb = (_OPEN(_open), _CLOSE(_close))
+            r = lib.mm_server_new(C.byref(self.options), self._cb[0], self._cb[1], None, C.byref(self._h))
+        if r != 0:
+            raise MontmulError(_host_error() or (lib.mm_server_last_error(None) or b"").decode())
+
+    def submit(self, n):
+        buf = to_limbs(int(n))
+        i = lib.mm_server_submit(self._h, buf, len(buf))
+        if i < 0:
+            raise MontmulError((lib.mm_server_last_error(self._h) or b"").decode())
+        return i
+
+    def poll(self, max_results=256, wait=0.0):
+        """Completed requests: dicts with id, prp (1, 0, or -1 on failure), len, bits and the
+        submit/admit/done times in seconds since the server started."""
+        out = (ServerResult * max_results)()
+        n = lib.mm_server_poll(self._h, out, max_results, wait)
+        return [{"id": r.id, "prp": r.prp, "len": r.len, "bits": r.bits, "submit_s": r.submit_s,
+                 "admit_s": r.admit_s, "done_s": r.done_s} for r in out[:n]]
+
+    def raw(self, request_id, max_limbs=1 << 20):
+        out = (C.c_uint32 * max_limbs)()
+        nl = C.c_size_t()
+        if lib.mm_server_raw(self._h, request_id, out, max_limbs, C.byref(nl)) != 0:
+            raise MontmulError((lib.mm_server_last_error(self._h) or b"").decode())
+        return int.from_bytes(C.string_at(out, 4 * nl.value), "little")
+
+    def stats(self):
+        st = ServerStats()
+        lib.mm_server_stats_get(self._h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib.mm_server_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -44,8 +44,11 @@ def build(verbose=False, out=LIB, extra=()):
     _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, "mm_hip.hip"), "-o", hip_obj], verbose)
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{GMP_INCLUDE}", "-c",
           os.path.join(CSRC, "mm_host.cpp"), "-o", host_obj], verbose)
+    serve_obj = os.path.join(OUT_DIR, "mm_serve.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{GMP_INCLUDE}", "-c",
+          os.path.join(CSRC, "mm_serve.cpp"), "-o", serve_obj], verbose)
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, host_obj,
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, host_obj, serve_obj,
           GMP_LIB, "-pthread"], verbose)
     os.replace(tmp, out)
     return out

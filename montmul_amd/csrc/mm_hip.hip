@@ -894,6 +894,7 @@ int mm_hip_open(const mm_options* opts, mm_backend* out) {
         return 1;
     }
     out->self = C;
+    out->set_modulus = mm_set_modulus;
     out->create = mm_create;
     out->destroy = mm_destroy;
     out->load = mm_load;
@@ -1043,6 +1044,53 @@ int mm_set_exponent(void* st, size_t i, const uint32_t* words, size_t nwords, ui
     if (nbits > 32ull * nwords || nbits > 0xffffffffull) return fail("nbits exceeds the exponent words");
     S->exps[i].assign(words, words + nwords);
     S->nbits[i] = (uint32_t)nbits;
+    S->pos[i] = 0;
+    if (!S->e_dirty[i]) {
+        S->e_dirty[i] = 1;
+        S->e_dirty_list.push_back((uint32_t)i);
+    }
+    return 0;
+}
+
+// Device constants of one slot for a new modulus: N'-hat, and A|B (or N-hat), as mm_create
+// computes them for every slot. The exponent is cleared.
+int mm_set_modulus(void* st, size_t i, const uint32_t* nl, const uint32_t* npl, const uint32_t* two_n) {
+    State* S = (State*)st;
+    Ctx* C = S->C;
+    (void)two_n;  // the device path has no use for 2N
+    if (i >= S->slots) return fail("slot out of range");
+    if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    const uint32_t k = S->k, n = S->n;
+    hipStream_t sm = C->stream;
+    uint32_t* lim = nullptr;
+    F1* t1 = nullptr;
+    F2* t2 = nullptr;
+    if (dalloc(&lim, 2ull * k)) return 1;
+    std::unique_ptr<uint32_t, void (*)(uint32_t*)> g0(lim, [](uint32_t* p) { hipFree(p); });
+    HIPCHK(hipMemcpyAsync(lim, npl, (size_t)k * 4, hipMemcpyHostToDevice, sm));
+    HIPCHK(hipMemcpyAsync(lim + k, nl, (size_t)k * 4, hipMemcpyHostToDevice, sm));
+    const uint32_t tt = plan_threads(n);
+    hipLaunchKernelGGL(k_load_forward, dim3(1), dim3(tt), 0, sm, S->tab->t, lim, k, S->d_nphat1 + i * n, S->d_nphat2 + i * n);
+    HIPCHK(hipGetLastError());
+    if (S->wrap) {
+        const uint32_t h = n / 2;
+        F1 c1 = cinv(F1{mod1(8 * h), 0});
+        F2 c2 = cinv(F2{(uint64_t)(8 * h), 0});
+        if (dalloc(&t1, n) || dalloc(&t2, n)) { hipFree(t1); return 1; }
+        hipLaunchKernelGGL(k_half_factors, dim3(1), dim3(plan_threads(h)), 0, sm, S->half->ht, lim + k, k, t1, t2,
+                           S->d_nhat1 + i * n, S->d_nhat2 + i * n, c1, c2);
+    } else {
+        hipLaunchKernelGGL(k_load_forward, dim3(1), dim3(tt), 0, sm, S->tab->t, lim + k, k, S->d_nhat1 + i * n,
+                           S->d_nhat2 + i * n);
+    }
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(sm);
+    hipFree(t1);
+    hipFree(t2);
+    if (le != hipSuccess) return hip_fail("set_modulus launch", le);
+    if (se != hipSuccess) return hip_fail("set_modulus", se);
+    S->exps[i].clear();
+    S->nbits[i] = 0;
     S->pos[i] = 0;
     if (!S->e_dirty[i]) {
         S->e_dirty[i] = 1;

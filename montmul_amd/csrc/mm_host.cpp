@@ -168,6 +168,40 @@ int mm_modbatch_new(const mm_backend* be, size_t count, const uint32_t* limbs, c
     return 0;
 }
 
+// Admission of another modulus into slot i (no reference counterpart; the reference builds a
+// new ModBatch): the setup values of mm_modbatch_new for one modulus, then the backend's
+// set_modulus.
+int mm_modbatch_set_modulus(mm_modbatch* b, size_t i, int sign, const uint32_t* limbs, size_t nl) {
+    if (i >= b->moduli.size()) return fail("slot " + std::to_string(i) + " out of range");
+    if (!b->be.set_modulus) return fail("this backend cannot replace a slot's modulus");
+    Mpz n, r, ninv, np, two;
+    from_signed_limbs(n.v, sign, limbs, nl);
+    if (mpz_sgn(n.v) <= 0 || mpz_even_p(n.v) || mpz_cmp_ui(n.v, 1) <= 0) return fail("modulus must be odd and > 1");
+    uint32_t pk, pl;
+    size_t bits = significant_bits(n.v);
+    if (bits > 0xffffffffu || mm_params_for_bits((uint32_t)bits, b->be.min_len, b->be.max_len, &pk, &pl)) return 1;
+    if (pl != b->len || pk > b->k)
+        return fail("modulus needs transform length " + std::to_string(pl) + ", the batch has " + std::to_string(b->len));
+    const uint32_t k = b->k;
+    mpz_setbit(r.v, 32 * (mp_bitcnt_t)k);
+    if (!mpz_invert(ninv.v, n.v, r.v)) return fail("modulus not invertible mod R");
+    mpz_sub(np.v, r.v, ninv.v);
+    mpz_mod(np.v, np.v, r.v);
+    mpz_mul_2exp(two.v, n.v, 1);
+    std::vector<uint32_t> nl_(k), npl(k), tnl(k);
+    if (!to_limbs(n.v, nl_.data(), k) || !to_limbs(np.v, npl.data(), k) || !to_limbs(two.v, tnl.data(), k))
+        return fail("value does not fit " + std::to_string(k) + " limbs");
+    MmModulus m;
+    mpz_set(m.n.v, n.v);
+    mpz_mod(m.r.v, r.v, n.v);
+    if (!mpz_invert(m.rinv.v, r.v, n.v)) return fail("R not invertible mod N");
+    if (b->be.set_modulus(b->state, i, nl_.data(), npl.data(), tnl.data())) return fail(b->be.last_error());
+    mpz_set(b->moduli[i].n.v, m.n.v);
+    mpz_set(b->moduli[i].r.v, m.r.v);
+    mpz_set(b->moduli[i].rinv.v, m.rinv.v);
+    return 0;
+}
+
 void mm_modbatch_free(mm_modbatch* b) { delete b; }
 
 void mm_modbatch_params(const mm_modbatch* b, uint32_t* k, uint32_t* len) { *k = b->k; *len = b->len; }

@@ -539,6 +539,27 @@ int oc_create(void* self, const mm_batch_setup* st, void** out) {
     return 0;
 }
 void oc_destroy(void* st) { delete (CpuState*)st; }
+// The table's set_modulus (an extension of this library's backend table, no counterpart in
+// cpu.rs): the slot's spectra as oc_create computes them, exponent cleared.
+int oc_set_modulus(void* st, size_t i, const uint32_t* n, const uint32_t* nprime, const uint32_t* two_n) {
+    CpuState* S = (CpuState*)st;
+    if (i >= S->slots.size()) return fail("slot out of range");
+    Slot& s = S->slots[i];
+    size_t nn = S->tab->n;
+    Scratch sc(nn);
+    if (S->half) {
+        half_nhat_of(n, *S->half, s.nhat1, s.nhat2);
+    } else {
+        forward_limbs(n, S->k, sc.a1.data(), sc.a2.data(), *S->tab);
+        s.nhat1 = sc.a1; s.nhat2 = sc.a2;
+    }
+    forward_limbs(nprime, S->k, sc.a1.data(), sc.a2.data(), *S->tab);
+    s.nphat1 = sc.a1; s.nphat2 = sc.a2;
+    s.two_n.assign(two_n, two_n + S->k);
+    s.exp.clear();
+    s.nbits = s.pos = 0;
+    return 0;
+}
 int oc_load(void* st, size_t i, const uint32_t* limbs) {  // cpu.rs:534-536
     CpuState* S = (CpuState*)st;
     if (i >= S->slots.size()) return fail("slot out of range");
@@ -757,6 +778,18 @@ int og_create(void* self, const mm_batch_setup* st, void** out) {  // gmp.rs:73-
     return 0;
 }
 void og_destroy(void* st) { delete (GmpState*)st; }
+// set_modulus for the Gmp restatement (extension, as oc_set_modulus).
+int og_set_modulus(void* st, size_t i, const uint32_t* n, const uint32_t* nprime, const uint32_t* two_n) {
+    GmpState* S = (GmpState*)st;
+    if (i >= S->slots.size()) return fail("slot out of range");
+    GmpSlot& s = *S->slots[i];
+    limbs_to_mpz(s.n, n, S->k);
+    limbs_to_mpz(s.nprime, nprime, S->k);
+    limbs_to_mpz(s.two_n, two_n, S->k);
+    s.exp.clear();
+    s.nbits = s.pos = 0;
+    return 0;
+}
 int og_load(void* st, size_t i, const uint32_t* limbs) {  // gmp.rs:89-91
     GmpState* S = (GmpState*)st;
     if (i >= S->slots.size()) return fail("slot out of range");

@@ -356,3 +356,58 @@ def transform_batched(t, max_len, seed=7):
                 b = int.from_bytes(cs[1][v * length:(v + 1) * length].tobytes(), "little")
                 assert val == a * b * (1 if scales is None else scales[v]), f"len {length} vector {v}"
         e += 1
+
+
+def set_modulus_admission(make, bits, seed=8):
+    """A slot reprogrammed for a new modulus of the same L mid-batch computes as a fresh batch
+    would (admission for serving): all slots 3^e mod N after the run, the replaced slot's
+    exponent cleared by the replacement, and a modulus of another L refused."""
+    rng = random.Random(seed)
+    moduli = [rand_odd(rng, bits) for _ in range(4)]
+    batch = ModBatch(make(), moduli, 3)
+    exps = [rng.getrandbits(96) | 1 for _ in moduli]
+    for i, e in enumerate(exps):
+        batch.load(i, 1)
+        batch.set_exponent(i, e)
+    batch.run(40)  # part of the way
+    n2 = rand_odd(rng, bits)
+    batch.set_modulus(2, n2)
+    assert batch.remaining(2) == 0 and batch.modulus(2) == n2
+    moduli[2] = n2
+    batch.load(2, 1)
+    batch.set_exponent(2, exps[2])
+    batch.run(1 << 20)
+    for i, (n, e) in enumerate(zip(moduli, exps)):
+        assert batch.remaining(i) == 0
+        assert batch.read(i) == pow(3, e, n), f"bits {bits} slot {i}"
+    try:
+        batch.set_modulus(1, rand_odd(rng, 4 * bits + 64))
+    except MontmulError:
+        pass
+    else:
+        raise AssertionError("a modulus of another transform length was accepted")
+
+
+def serve_mix(server, make, moduli, base=3, timeout_s=120.0):
+    """Submits every modulus to `server` (a montmul_amd.Server with keep_raw), waits for all,
+    and checks each result: prp == [base^(N-1) = 1 mod N] and the final raw limbs equal those of
+    the same test run alone in a batch of its own on `make()` (alone-vs-mix equality)."""
+    import time
+
+    ids = {server.submit(n): n for n in moduli}
+    got, t_end = {}, time.time() + timeout_s
+    while len(got) < len(ids) and time.time() < t_end:
+        for r in server.poll(256, 0.2):
+            got[r["id"]] = r
+    assert len(got) == len(ids), f"{len(ids) - len(got)} requests unfinished"
+    for i, n in ids.items():
+        r = got[i]
+        assert r["prp"] == int(pow(base, n - 1, n) == 1), f"request {i}: prp {r['prp']}"
+        assert r["bits"] == (n - 1).bit_length()
+        assert r["submit_s"] <= r["admit_s"] <= r["done_s"]
+        alone = ModBatch(make(), [n], base)
+        alone.load(0, 1)
+        alone.set_exponent(0, n - 1)
+        alone.run(1 << 30)
+        assert server.raw(i) == alone.read_raw(0), f"request {i}: raw differs alone and in the mix"
+    return got

@@ -75,6 +75,7 @@ def _table(prefix, cfg, bound, min_len, max_len, full_k):
     t.mul = mm._MUL((prefix + "mul", olib))
     t.read = mm._READ((prefix + "read", olib))
     t.remaining = mm._REMAINING((prefix + "remaining", olib))
+    t.set_modulus = mm._SETMOD((prefix + "set_modulus", olib))
     t.last_error = mm._LASTERR(("oc_last_error", olib))
     t.bound_lo, t.bound_hi = bound & (2**64 - 1), bound >> 64
     t.min_len, t.max_len, t.full_k = min_len, max_len, int(full_k)

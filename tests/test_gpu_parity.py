@@ -283,3 +283,11 @@ def test_cpu_case_4096_bits(hip):
         assert b.remaining(i) == 0
         assert b.read(i) == pow(3, e, n), f"slot {i}"
         assert b.read_raw(i) == ref.read_raw(i), f"slot {i} raw"
+
+
+def test_set_modulus(hip, hip_global):
+    """Admission of a new modulus into a running batch, both modes and both kernels' lengths."""
+    for be in (hip, hip_global):
+        for bits in (60, 1000, 5000, 100_000):
+            common.set_modulus_admission(lambda: be, bits)
+    common.set_modulus_admission(lambda: Hip(wraparound=False), 1000)

@@ -5,6 +5,7 @@ that finish in seconds."""
 import pytest
 
 import common
+import montmul_amd as mm
 from montmul_amd import MontmulError, ModBatch, fermat
 from oracle_backends import Cpu, Gmp
 
@@ -95,3 +96,23 @@ def test_params_rules():
     assert Params.for_bits(4096, lengths) == Params(129, 512)
     b = ModBatch(Cpu(wraparound=True), [(1 << 99_999) | 1], 3)
     assert b.params() == Params(4096, 8192)
+
+
+def test_server_on_oracle():
+    """The request server over length pools, with the oracle's Cpu backend (wraparound) for
+    every pool: mixed lengths and a pool smaller than the load, so slots are reused; every result
+    equals base^(N-1) = 1 mod N and its raw limbs equal a run alone."""
+    import random
+
+    from oracle_backends import Cpu
+
+    rng = random.Random(12)
+    moduli = [common.rand_odd(rng, b) for b in (60, 64, 100, 500, 1000, 1000, 2000, 3000)]
+    moduli += [2**127 - 1, 2**521 - 1, 2**607 - 1, common.tribonacci(87)]  # primes
+    srv = mm.Server(base=3, pool_slots=2, target_ms=5, keep_raw=True, backend_factory=lambda: Cpu(wraparound=True))
+    try:
+        common.serve_mix(srv, lambda: Cpu(wraparound=True), moduli)
+        st = srv.stats()
+        assert st["completed"] == len(moduli) and st["failed"] == 0 and st["pools"] >= 3
+    finally:
+        srv.close()

@@ -239,3 +239,14 @@ def test_cpu_case_4096_bits_oracle():
     for i, (n, e) in enumerate(zip(moduli, exps)):
         assert a.read(i) == pow(3, e, n), f"slot {i}"
         assert a.read_raw(i) == b.read_raw(i), f"slot {i} raw"
+
+
+@pytest.mark.parametrize("wrap", [True, False])
+def test_set_modulus_oracle(wrap):
+    """Admission of a new modulus into a running batch, on the Cpu restatement."""
+    for bits in (60, 1000, 5000):
+        common.set_modulus_admission(lambda: Cpu(wraparound=wrap), bits)
+
+
+def test_set_modulus_gmp_oracle():
+    common.set_modulus_admission(lambda: Gmp(), 1000)
