@@ -116,3 +116,17 @@ def test_server_on_oracle():
         assert st["completed"] == len(moduli) and st["failed"] == 0 and st["pools"] >= 3
     finally:
         srv.close()
+
+
+def test_cli_arguments():
+    """The command line's option rules (main.rs:32-66): unknown, repeated and value-less keys
+    are errors; values parse as numbers where needed."""
+    from montmul_amd.__main__ import Args
+
+    a = Args("bench", ["--bits", "1000", "--slots", "4", "--steps", "10", "--bare"])
+    assert a.num("bits") == 1000 and a.flag("bare") and not a.flag("global")
+    for bad in (["--bits", "1", "--bits", "2"], ["--nope", "1"], ["--bits"], ["--bits", "--slots"], ["bits"]):
+        with pytest.raises(SystemExit):
+            Args("bench", bad)
+    with pytest.raises(SystemExit):
+        Args("prp", ["--index", "x"]).num("index")
