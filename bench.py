@@ -182,7 +182,7 @@ def main():
     modes = [m for m in args.modes.split(",") if m]
     results = {}
     for mode in modes:
-        hip = mm.Hip(force_global=(mode == "global"))
+        hip = mm.Hip(force_global=(mode == "global"), prefer_lds=(mode == "lds"))
         cus = hip.cu_count()
         if args.slots:
             best, sw = args.slots, {}

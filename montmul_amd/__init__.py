@@ -76,7 +76,7 @@ _CLOSE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(BackendTable))
 
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("force_global", C.c_int32), ("wraparound", C.c_int32),
-                ("max_launch_ms", C.c_uint32)]
+                ("max_launch_ms", C.c_uint32), ("prefer_lds", C.c_int32)]
 
 
 class ServerOptions(C.Structure):
@@ -246,8 +246,9 @@ class Backend:
 class Hip(Backend):
     """The MI355X backend (replaces `Gpu`/`GpuOptions`, gpu/mod.rs:28-111)."""
 
-    def __init__(self, device=0, force_global=False, wraparound=True, max_launch_ms=0):
-        self.options = Options(device, int(bool(force_global)), int(bool(wraparound)), max_launch_ms)
+    def __init__(self, device=0, force_global=False, wraparound=True, max_launch_ms=0, prefer_lds=False):
+        self.options = Options(device, int(bool(force_global)), int(bool(wraparound)), max_launch_ms,
+                               int(bool(prefer_lds)))
         self.table = BackendTable()
         if lib.mm_hip_open(C.byref(self.options), C.byref(self.table)) != 0:
             raise MontmulError((lib.mm_last_error() or b"").decode())
@@ -600,7 +601,7 @@ class Server:
         self._h = C.c_void_p()
         self._keep = []
         if backend_factory is None:
-            hip = Options(device, int(bool(force_global)), 1, 0)
+            hip = Options(device, int(bool(force_global)), 1, 0, 0)
             r = lib.mm_server_new_hip(C.byref(self.options), C.byref(hip), C.byref(self._h))
         else:
             def _open(user, out):

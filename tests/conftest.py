@@ -16,10 +16,10 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="module")
 def hip():
-    """The HIP backend on device 0 (LDS or global mode as planned)."""
+    """The HIP backend on device 0, vectors in LDS wherever they fit (the reference planner's rule)."""
     from montmul_amd import Hip
 
-    h = Hip()
+    h = Hip(prefer_lds=True)
     yield h
     h.close()
 

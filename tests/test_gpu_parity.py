@@ -247,10 +247,12 @@ def test_headline_shape_closed_form(hip, hip_global):
         ref.load(i, 1)
         ref.set_exponent(i, e)
     ref.run(16)
-    for be, name in ((hip, "lds"), (hip_global, "global")):
+    auto = Hip()  # the default planner: global mode at L = 8,192 for two workgroups per CU
+    for be, name in ((hip, "lds"), (hip_global, "global"), (auto, "default")):
         say("headline shape", name)
         b = ModBatch(be, moduli, 3)
         assert b.params().k == 4096 and b.params().len == 8192
+        assert Hip.plan_of(b)["global"] == (name != "lds"), name
         for i, e in enumerate(exps):
             b.load(i, 1)
             b.set_exponent(i, e)
@@ -258,6 +260,7 @@ def test_headline_shape_closed_form(hip, hip_global):
         bad = [i for i in range(256) if b.read_raw(i) != ref.read_raw(i)]
         assert not bad, f"{name}: slots {bad[:8]} differ"
         b.free()
+    auto.close()
 
 
 def test_cpu_case_4096_bits(hip):
