@@ -94,8 +94,9 @@ typedef struct mm_options {
     int32_t wraparound;      /* reduction through the half-length cyclic product; k = L/2 */
     uint32_t max_launch_ms;  /* target duration of one kernel launch; 0 = no splitting */
     int32_t prefer_lds;      /* keep vectors in LDS whenever they fit (the reference planner's
-                              * rule, plan.rs:26-42); by default global mode is also chosen where
-                              * it doubles the workgroups per CU */
+                              * rule, plan.rs:26-42); by default the register-transform kernel
+                              * (wraparound, L = 1,024 .. 8,192) runs in global mode, whose smaller
+                              * LDS footprint fits more workgroups per CU and is faster there */
 } mm_options;
 
 /* Launch plan of a batch. Replaces `LaunchPlan`/`Mode` (gpu/plan.rs:5-22). */

@@ -1,6 +1,6 @@
 """Command-line front end (reference: montmul/src/main.rs):
 
-    python -m montmul_amd bench --bits B --slots S --steps K [--bare] [--global] [--csv PATH]
+    python -m montmul_amd bench --bits B --slots S --steps K [--bare] [--global | --lds] [--csv PATH]
     python -m montmul_amd prp --index N
     python -m montmul_amd score [--slots S] [--label TEXT] [--rocprof] [--csv PATH]
     python -m montmul_amd fermat --bits B --slots S [--seed N] [--planted FILE] [--planted-slots P]
@@ -25,10 +25,10 @@ import montmul_amd as mm
 from montmul_amd import score as sc
 
 ALLOWED = {  # main.rs:21-30, plus fermat (the full-test workload) and --global
-    "bench": (["backend", "bits", "slots", "steps", "csv"], ["bare", "global"]),
+    "bench": (["backend", "bits", "slots", "steps", "csv"], ["bare", "global", "lds"]),
     "prp": (["index", "backend"], []),
     "score": (["backend", "slots", "label", "csv"], ["rocprof"]),
-    "fermat": (["backend", "bits", "slots", "seed", "planted", "planted-slots", "check", "csv"], ["global"]),
+    "fermat": (["backend", "bits", "slots", "seed", "planted", "planted-slots", "check", "csv"], ["global", "lds"]),
     "rocprof-launch": (["slots"], []),
 }
 
@@ -108,7 +108,7 @@ def cmd_bench(args):
     bits, slots, steps = args.num("bits"), args.num("slots"), args.num("steps")
     if bits < 2 or slots <= 0 or steps <= 0 or steps > bits:
         raise SystemExit("bits >= 2, slots > 0 and 0 < steps <= bits required")
-    hip = mm.Hip(force_global=args.flag("global"))
+    hip = mm.Hip(force_global=args.flag("global"), prefer_lds=args.flag("lds"))
     moduli, exps = mm.score_workload(bits, slots, steps, 0)
     b = mm.ModBatch(hip, moduli, 3)
     run = (lambda k: mm.Hip.run_bare(b, k)) if args.flag("bare") else b.run
@@ -232,7 +232,7 @@ def cmd_fermat(args):
     planted = min(args.num("planted-slots", len(primes) and slots // 2), slots) if primes else 0
     rand, _ = mm.score_workload(bits, slots - planted, bits, seed) if slots > planted else ([], [])
     moduli = [primes[i % len(primes)] for i in range(planted)] + rand
-    hip = mm.Hip(force_global=args.flag("global"))
+    hip = mm.Hip(force_global=args.flag("global"), prefer_lds=args.flag("lds"))
     b = mm.ModBatch(hip, moduli, 3)
     for i, n in enumerate(moduli):
         b.load(i, 1)

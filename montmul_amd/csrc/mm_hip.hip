@@ -946,12 +946,12 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
     if (S->v2) {
         S->threads = n / 8;
         const uint64_t resident = v2::v2_lds_resident(n, k, S->threads);
-        // Global mode also when the resident layout leaves one workgroup per CU while the tiled
-        // one allows two (L = 8,192: 131 KiB resident against 49 KiB tiled, 2.3 M against
-        // 3.3 M squarings/s): occupancy beats keeping T in LDS.
-        const uint64_t tiled = 24ull * std::min<uint32_t>(n, 2048) + lds_scratch_bytes(n / 8);
-        S->global = C->opts.force_global || resident > C->max_dyn_lds_v2 ||
-                    (!C->opts.prefer_lds && 2 * resident > C->max_lds && 2 * tiled <= C->max_lds);
+        // Global mode unless LDS is preferred: its smaller LDS footprint (T in HBM, exchange
+        // tiles of at most 2,048 elements) fits more workgroups per CU, and that wins at every
+        // length of this kernel (squarings x bits per second, 4,096 slots: L = 1,024: 3.73e11
+        // against 3.21e11 resident; 2,048: 3.41 against 2.96; 4,096: 3.74 against 3.22;
+        // 8,192: 3.3 M against 2.3 M squarings/s).
+        S->global = C->opts.force_global || resident > C->max_dyn_lds_v2 || !C->opts.prefer_lds;
         const uint64_t tile = std::min<uint32_t>(n, 2048);
         S->lds = S->global ? 24 * tile + lds_scratch_bytes(S->threads) : resident;
     } else {

@@ -37,3 +37,14 @@ def test_cli_fermat_planted(tmp_path):
     primes.write_text("%x\n%x\n" % (2**4253 - 1, 2**4423 - 1))
     f = cli("fermat", "--bits", "4253", "--slots", "32", "--planted", str(primes), "--planted-slots", "8", "--check", "4")
     assert f["planted"] == "8" and f["planted_not_prime"] == "0" and f["powm_mismatch"] == "0"
+
+
+def test_cli_fermat_20000_bits():
+    """The full-test workload at a reduced slot count: 20,000-bit moduli (L = 2,048), e = N - 1,
+    half the slots cycling through the planted primes of tests/golden/planted_20000bit.txt, the
+    others from GMP's generator seeded 7; planted primes come out prime, sampled others match
+    GMP powm."""
+    planted = os.path.join(ROOT, "tests", "golden", "planted_20000bit.txt")
+    f = cli("fermat", "--bits", "20000", "--slots", "256", "--seed", "7", "--planted", planted,
+            "--planted-slots", "128", "--check", "4")
+    assert f["L"] == "2048" and f["planted"] == "128" and f["planted_not_prime"] == "0" and f["powm_mismatch"] == "0"

@@ -247,7 +247,7 @@ def test_headline_shape_closed_form(hip, hip_global):
         ref.load(i, 1)
         ref.set_exponent(i, e)
     ref.run(16)
-    auto = Hip()  # the default planner: global mode at L = 8,192 for two workgroups per CU
+    auto = Hip()  # the default planner: global mode for the register-transform kernel
     for be, name in ((hip, "lds"), (hip_global, "global"), (auto, "default")):
         say("headline shape", name)
         b = ModBatch(be, moduli, 3)
