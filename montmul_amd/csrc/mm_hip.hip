@@ -75,7 +75,7 @@ __global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_
     }
 }
 
-// The same steps on register-resident transforms (wraparound path, n = 512 .. 4,096): the
+// The same steps on register-resident transforms (wraparound path, n = 512 .. 8,192): the
 // vector lives in registers, exchanges go through LDS; in global mode T and x stay in HBM and
 // the exchanges use an LDS tile of at most 2,048 elements. The kernels are held to 128 VGPRs
 // (four waves per SIMD), so two 512-thread workgroups share a CU wherever LDS allows.
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_
 #define MM_V2_WAVES_PER_EU 4
 #endif
 template <uint32_t LOGN, bool GLOBAL, bool BARE>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_PER_EU))) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
+__global__ __launch_bounds__(LOGN > 12 ? 1024 : 512) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_PER_EU))) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
     extern __shared__ __align__(16) unsigned char lds[];
     const Slots& S = A.s;
     const uint32_t slot = first + blockIdx.x;
@@ -668,7 +668,8 @@ StepKernelV2 pick_step_v2(uint32_t logn, bool global, bool bare) {
         case 9: return pick_v2<9>(global, bare);
         case 10: return pick_v2<10>(global, bare);
         case 11: return pick_v2<11>(global, bare);
-        default: return pick_v2<12>(global, bare);
+        case 12: return pick_v2<12>(global, bare);
+        default: return pick_v2<13>(global, bare);
     }
 }
 StepKernel pick_step(bool global, bool wrap, bool bare) {
@@ -697,7 +698,7 @@ int set_lds_limit(Ctx* C) {
         }
         dyn_max = std::min(dyn_max, dyn);
     }
-    for (uint32_t i = 0; i < 16; i++) {
+    for (uint32_t i = 0; i < 20; i++) {
         StepKernelV2 k = pick_step_v2(9 + i / 4, (i >> 1) & 1, i & 1);
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, (const void*)k));
@@ -942,7 +943,7 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
     S->C = C;
     S->k = k; S->len = st->len; S->n = n; S->base = st->base; S->slots = st->slots; S->wrap = wrap;
     const char* kenv = getenv("MONTMUL_KERNEL");
-    S->v2 = wrap && n >= 512 && n <= 4096 && !(kenv && strcmp(kenv, "v1") == 0);
+    S->v2 = wrap && n >= 512 && n <= 8192 && !(kenv && strcmp(kenv, "v1") == 0);
     if (S->v2) {
         S->threads = n / 8;
         const uint64_t resident = v2::v2_lds_resident(n, k, S->threads);
@@ -952,7 +953,7 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         // against 3.21e11 resident; 2,048: 3.41 against 2.96; 4,096: 3.74 against 3.22;
         // 8,192: 3.3 M against 2.3 M squarings/s).
         S->global = C->opts.force_global || resident > C->max_dyn_lds_v2 || !C->opts.prefer_lds;
-        const uint64_t tile = std::min<uint32_t>(n, 2048);
+        const uint64_t tile = std::min<uint32_t>(n, n > 4096 ? 4096 : 2048);  // v2::Tiling
         S->lds = S->global ? 24 * tile + lds_scratch_bytes(S->threads) : resident;
     } else {
         S->threads = plan_threads(n);

@@ -375,11 +375,14 @@ template <u32 LOGN> struct CPlan {
 };
 
 // Largest exchange tile in elements: with TILED (global mode) the exchange buffer holds at
-// most 2,048 elements of both fields (48 KiB), and a longer vector is exchanged in chunks.
-constexpr u32 TILE_MAX = 2048;
+// most 2,048 elements of both fields (48 KiB; two workgroups per CU up to n = 4,096), and a
+// longer vector is exchanged in chunks. At n = 8,192 the 1,024-thread workgroup has a CU to
+// itself, so its tile is 4,096 elements (96 KiB): two chunks, like n = 4,096, where four would
+// hold twice the registers in flight.
+constexpr u32 tile_max(u32 logn) { return logn > 12 ? 4096 : 2048; }
 template <u32 LOGN, bool TILED> struct Tiling {
     static constexpr u32 n = 1u << LOGN;
-    static constexpr u32 tile = (TILED && n > TILE_MAX) ? TILE_MAX : n;
+    static constexpr u32 tile = (TILED && n > tile_max(LOGN)) ? tile_max(LOGN) : n;
     static constexpr u32 chunks = n / tile;
 };
 
