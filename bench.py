@@ -103,32 +103,32 @@ def sweep(mm, hip, moduli, exps, candidates, sweep_ebits):
     return best, res
 
 
-def cpu_baseline(bits, ebits, threads):
+def cpu_baseline(bits, threads, gmp_steps=24_000, ntt_steps=4_000):
     """The oracle's Gmp restatement (closed-form REDC on GMP integers, gmp.rs) on this host's
-    cores, threads x 1 slots, and its Cpu (NTT) restatement on a short sample."""
+    cores, `threads` slots of the score workload for `gmp_steps` steps (about 10 s), and its Cpu
+    (NTT, wraparound) restatement for `ntt_steps` steps (about 5 s)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import montmul_amd as mm
     from oracle_backends import Cpu, Gmp
 
-    moduli, exps = mm.score_workload(bits, threads, ebits, 2026)
+    moduli, exps = mm.score_workload(bits, threads, max(gmp_steps, ntt_steps), 2026)
     out = {}
     b = mm.ModBatch(Gmp(threads=threads), moduli, 3)
     arm(b, exps)
     b.run(16)
     arm(b, exps)
     t0 = time.perf_counter()
-    b.run(ebits)
+    b.run(gmp_steps)
     dt = time.perf_counter() - t0
-    out["gmp"] = threads * ebits / dt
-    steps = 64
+    out["gmp"] = threads * gmp_steps / dt
     c = mm.ModBatch(Cpu(wraparound=True, threads=threads), moduli, 3)
     arm(c, exps)
     t0 = time.perf_counter()
-    c.run(steps)
+    c.run(ntt_steps)
     dt2 = time.perf_counter() - t0
-    out["cpu_ntt"] = threads * steps / dt2
-    out["gmp_sample"] = f"{threads} slots x {ebits} steps, {bits}-bit moduli, {dt:.2f} s wall"
-    out["cpu_sample"] = f"{threads} slots x {steps} steps, {bits}-bit moduli, {dt2:.2f} s wall"
+    out["cpu_ntt"] = threads * ntt_steps / dt2
+    out["gmp_sample"] = f"{threads} slots x {gmp_steps} steps, {bits}-bit moduli, {dt:.2f} s wall"
+    out["cpu_sample"] = f"{threads} slots x {ntt_steps} steps, {bits}-bit moduli, {dt2:.2f} s wall"
     return out
 
 
@@ -257,7 +257,7 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
-        cb = cpu_baseline(args.bits, ebits, threads)
+        cb = cpu_baseline(args.bits, threads)
         line["cpu_baseline"] = {"value": round(cb["gmp"], 1), "unit": "squarings/s", "cores": threads,
                                 "kind": "port", "sample": "oracle Gmp restatement (gmp.rs closed-form REDC): " + cb["gmp_sample"],
                                 "cpu_ntt_value": round(cb["cpu_ntt"], 1),
