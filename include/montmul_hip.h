@@ -73,6 +73,12 @@ typedef struct mm_backend {
      * backend cannot. No reference counterpart: it admits new work into a running batch
      * (serving) without rebuilding the batch. */
     int (*set_modulus)(void *state, size_t slot, const uint32_t *n, const uint32_t *nprime, const uint32_t *two_n);
+    /* Runs up to `bits` steps on every slot with bits left in `count` states of this backend
+     * together (typically one per length), as `run` on each would; returns MM_OK, the first
+     * failing slot with its state's index in *which, or MM_EFAULT. NULL when the backend has
+     * no joint run (the host then runs the states one after another). No reference
+     * counterpart: the reference's batches hold one length each (batch.rs:64-77). */
+    int64_t (*run_group)(void *const *states, size_t count, uint64_t bits, size_t *which);
     /* Message of the calling thread's last error of this backend. */
     const char *(*last_error)(void);
     /* Limits::coefficient_bound (transform.rs:13), as a 128-bit value. */
@@ -140,6 +146,11 @@ uint64_t mm_max_lds(void *self);
 int mm_device_name(void *self, char *buf, size_t len);
 /* The Hip table's set_modulus (see mm_backend): device constants of one slot rebuilt. */
 int mm_set_modulus(void *state, size_t slot, const uint32_t *n, const uint32_t *nprime, const uint32_t *two_n);
+/* The Hip table's run_group (see mm_backend): the states' slots are stepped by one persistent
+ * launch of the mixed-length kernel per chunk. Its workgroups take per-slot descriptors (length
+ * class, slot) from a queue in device memory, longest work first, and step each slot at its own
+ * length. Every state must come from the same mm_hip_open. */
+int64_t mm_run_group(void *const *states, size_t count, uint64_t bits, size_t *which);
 /* Device time of the step kernel launches of the latest run, in milliseconds (HIP events on
  * the batch's stream), and the number of launches. No reference counterpart: measurement. */
 int mm_last_run_kernel_ms(void *state, double *ms, uint32_t *launches);
@@ -230,6 +241,31 @@ int mm_modbatch_modulus(mm_modbatch *b, size_t slot, uint32_t *out);
  * rebuilds the slot; its exponent is cleared. Fails when the backend has no set_modulus. No
  * reference counterpart (the reference rebuilds a ModBatch): admission for serving. */
 int mm_modbatch_set_modulus(mm_modbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
+
+/* A batch of moduli needing different transform lengths (no reference counterpart: ModBatch::new
+ * rejects mixed lengths, batch.rs:64-77). Each modulus gets the Params of its own size
+ * (batch.rs:20-25); the moduli of one length form one ModBatch, and run() steps all of them
+ * together through the backend's run_group (one after another if it has none). Slot i is the
+ * i-th modulus given; the per-slot functions behave as the mm_modbatch ones with that slot's
+ * k limbs, and run returns MM_OK, the first failing slot of this batch, or MM_EFAULT. */
+typedef struct mm_mixbatch mm_mixbatch;
+int mm_mixbatch_new(const mm_backend *be, size_t count, const uint32_t *limbs, const size_t *offsets,
+                    const int8_t *signs, uint32_t base, mm_mixbatch **out);
+void mm_mixbatch_free(mm_mixbatch *b);
+size_t mm_mixbatch_slots(const mm_mixbatch *b);
+size_t mm_mixbatch_groups(const mm_mixbatch *b);
+int mm_mixbatch_params(const mm_mixbatch *b, size_t slot, uint32_t *k, uint32_t *len);
+int mm_mixbatch_load(mm_mixbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
+int mm_mixbatch_load_raw(mm_mixbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
+int mm_mixbatch_set_exponent(mm_mixbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
+int64_t mm_mixbatch_run(mm_mixbatch *b, uint64_t bits);
+uint64_t mm_mixbatch_remaining(mm_mixbatch *b, size_t slot);
+int mm_mixbatch_read_raw(mm_mixbatch *b, size_t slot, uint32_t *out);
+int mm_mixbatch_read(mm_mixbatch *b, size_t slot, uint32_t *out);
+int mm_mixbatch_is_one(mm_mixbatch *b, size_t slot);
+int mm_mixbatch_check_pow(mm_mixbatch *b, size_t slot, uint32_t base, const uint32_t *e, size_t ne);
+/* The backend state of group `group` (groups in ascending L), NULL if out of range. */
+void *mm_mixbatch_group_state(mm_mixbatch *b, size_t group);
 
 /* fermat (batch.rs:187-198): base-`base` Fermat PRP test of every modulus; prp[i] = 1 or 0. */
 int mm_fermat(const mm_backend *be, size_t count, const uint32_t *limbs, const size_t *offsets,

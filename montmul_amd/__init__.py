@@ -58,6 +58,7 @@ _READ = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32))
 _REMAINING = C.CFUNCTYPE(C.c_uint64, C.c_void_p, C.c_size_t)
 _SETMOD = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                       C.POINTER(C.c_uint32))
+_RUNGROUP = C.CFUNCTYPE(C.c_int64, C.POINTER(C.c_void_p), C.c_size_t, C.c_uint64, C.POINTER(C.c_size_t))
 _LASTERR = C.CFUNCTYPE(C.c_char_p)
 
 
@@ -65,7 +66,8 @@ class BackendTable(C.Structure):
     """`mm_backend`: the backend as a table of functions (transform.rs:11-20, 61-77)."""
     _fields_ = [("self", C.c_void_p), ("create", _CREATE), ("destroy", _DESTROY), ("load", _LOAD),
                 ("set_exponent", _SETEXP), ("run", _RUN), ("mul", _MUL), ("read", _READ),
-                ("remaining", _REMAINING), ("set_modulus", _SETMOD), ("last_error", _LASTERR),
+                ("remaining", _REMAINING), ("set_modulus", _SETMOD), ("run_group", _RUNGROUP),
+                ("last_error", _LASTERR),
                 ("bound_lo", C.c_uint64), ("bound_hi", C.c_uint64),
                 ("min_len", C.c_uint32), ("max_len", C.c_uint32), ("full_k", C.c_int32)]
 
@@ -153,6 +155,23 @@ def _load_lib():
                                 C.POINTER(C.c_int8), C.c_uint32, C.c_uint64, C.POINTER(C.c_uint8)]),
         "mm_score_workload": (C.c_int, [C.c_uint32, C.c_size_t, C.c_uint32, C.c_uint64, u32p, u32p]),
         "mm_modbatch_check_pow": (C.c_int, [vp, C.c_size_t, C.c_uint32, u32p, C.c_size_t]),
+        "mm_run_group": (C.c_int64, [C.POINTER(vp), C.c_size_t, C.c_uint64, C.POINTER(C.c_size_t)]),
+        "mm_mixbatch_new": (C.c_int, [C.POINTER(BackendTable), C.c_size_t, u32p, C.POINTER(C.c_size_t),
+                                      C.POINTER(C.c_int8), C.c_uint32, C.POINTER(vp)]),
+        "mm_mixbatch_free": (None, [vp]),
+        "mm_mixbatch_slots": (C.c_size_t, [vp]),
+        "mm_mixbatch_groups": (C.c_size_t, [vp]),
+        "mm_mixbatch_params": (C.c_int, [vp, C.c_size_t, u32p, u32p]),
+        "mm_mixbatch_load": (C.c_int, [vp, C.c_size_t, C.c_int, u32p, C.c_size_t]),
+        "mm_mixbatch_load_raw": (C.c_int, [vp, C.c_size_t, C.c_int, u32p, C.c_size_t]),
+        "mm_mixbatch_set_exponent": (C.c_int, [vp, C.c_size_t, C.c_int, u32p, C.c_size_t]),
+        "mm_mixbatch_run": (C.c_int64, [vp, C.c_uint64]),
+        "mm_mixbatch_remaining": (C.c_uint64, [vp, C.c_size_t]),
+        "mm_mixbatch_read_raw": (C.c_int, [vp, C.c_size_t, u32p]),
+        "mm_mixbatch_read": (C.c_int, [vp, C.c_size_t, u32p]),
+        "mm_mixbatch_is_one": (C.c_int, [vp, C.c_size_t]),
+        "mm_mixbatch_check_pow": (C.c_int, [vp, C.c_size_t, C.c_uint32, u32p, C.c_size_t]),
+        "mm_mixbatch_group_state": (vp, [vp, C.c_size_t]),
         "mm_t_upload": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
         "mm_t_load_forward": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
         "mm_t_download": (C.c_int, [vp, u64p]),
@@ -562,6 +581,105 @@ class ModBatch:
 
     def is_one(self, i):
         r = lib.mm_modbatch_is_one(self._h, i)
+        if r < 0:
+            raise MontmulError(_host_error())
+        return bool(r)
+
+
+class MixBatch:
+    """Moduli of different sizes in one batch: each gets the Params of its own size
+    (batch.rs:20-25), the moduli of one length form one ModBatch, and run() steps every length
+    together, through one persistent launch on the Hip backend (group by group on a backend
+    without a joint run). Same per-slot methods as ModBatch. No reference counterpart:
+    ModBatch::new rejects mixed lengths (batch.rs:64-77)."""
+
+    def __init__(self, backend, moduli, base):
+        self.backend = backend
+        moduli = [int(m) for m in moduli]
+        limbs, offsets, signs = _pack_ints(moduli)
+        h = C.c_void_p()
+        if lib.mm_mixbatch_new(C.byref(backend.table), len(moduli), limbs, offsets, signs, base, C.byref(h)) != 0:
+            raise MontmulError(_host_error())
+        self._h = h
+        self._moduli = moduli
+        self._params = []
+        for i in range(len(moduli)):
+            k, ln = C.c_uint32(), C.c_uint32()
+            lib.mm_mixbatch_params(h, i, C.byref(k), C.byref(ln))
+            self._params.append(Params(k.value, ln.value))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mm_mixbatch_free(h)
+            self._h = None
+
+    def free(self):
+        self.__del__()
+
+    def params(self, i):
+        if not 0 <= i < len(self._params):
+            raise MontmulError(f"slot {i} out of range")
+        return self._params[i]
+
+    def slots(self):
+        return len(self._moduli)
+
+    def groups(self):
+        return lib.mm_mixbatch_groups(self._h)
+
+    def group_state(self, g):
+        """The backend state of length group g (ascending L)."""
+        return lib.mm_mixbatch_group_state(self._h, g)
+
+    def modulus(self, i):
+        return self._moduli[i]
+
+    def load(self, i, x):
+        x = int(x)
+        buf = to_limbs(x)
+        if lib.mm_mixbatch_load(self._h, i, _sign(x), buf, len(buf)) != 0:
+            raise MontmulError(_host_error())
+
+    def load_raw(self, i, xm):
+        xm = int(xm)
+        buf = to_limbs(xm)
+        if lib.mm_mixbatch_load_raw(self._h, i, _sign(xm), buf, len(buf)) != 0:
+            raise MontmulError(_host_error())
+
+    def set_exponent(self, i, e):
+        e = int(e)
+        buf = to_limbs(e)
+        if lib.mm_mixbatch_set_exponent(self._h, i, _sign(e), buf, len(buf)) != 0:
+            raise MontmulError(_host_error())
+
+    def run(self, bits):
+        _check_run(lib.mm_mixbatch_run(self._h, bits), _host_error)
+
+    def remaining(self, i):
+        return lib.mm_mixbatch_remaining(self._h, i)
+
+    def read_raw(self, i):
+        buf = (C.c_uint32 * self.params(i).k)()
+        if lib.mm_mixbatch_read_raw(self._h, i, buf) != 0:
+            raise MontmulError(_host_error())
+        return from_limbs(buf)
+
+    def read(self, i):
+        buf = (C.c_uint32 * self.params(i).k)()
+        if lib.mm_mixbatch_read(self._h, i, buf) != 0:
+            raise MontmulError(_host_error())
+        return from_limbs(buf)
+
+    def check_pow(self, i, base, e):
+        buf = to_limbs(int(e))
+        r = lib.mm_mixbatch_check_pow(self._h, i, base, buf, len(buf))
+        if r < 0:
+            raise MontmulError(_host_error())
+        return bool(r)
+
+    def is_one(self, i):
+        r = lib.mm_mixbatch_is_one(self._h, i)
         if r < 0:
             raise MontmulError(_host_error())
         return bool(r)

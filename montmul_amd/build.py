@@ -37,18 +37,42 @@ def _run(cmd, verbose):
         print(out)
 
 
+def _run_all(cmds, verbose):
+    """Runs the commands in parallel; fails with the first failing one's output."""
+    procs = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = None
+    for cmd, p in procs:
+        text = p.communicate()[0]
+        out = "\n".join(l for l in text.splitlines() if "packed-fp32-ops" not in l)
+        if p.returncode != 0 and failed is None:
+            failed = f"build step failed ({p.returncode}): {' '.join(cmd)}\n{out}"
+        elif verbose and out.strip():
+            print(out)
+    if failed:
+        raise RuntimeError(failed)
+
+
 def build(verbose=False, out=LIB, extra=()):
     os.makedirs(OUT_DIR, exist_ok=True)
     hip_obj = os.path.join(OUT_DIR, "mm_hip.o")
     host_obj = os.path.join(OUT_DIR, "mm_host.o")
-    _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, "mm_hip.hip"), "-o", hip_obj], verbose)
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{GMP_INCLUDE}", "-c",
-          os.path.join(CSRC, "mm_host.cpp"), "-o", host_obj], verbose)
     serve_obj = os.path.join(OUT_DIR, "mm_serve.o")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{GMP_INCLUDE}", "-c",
-          os.path.join(CSRC, "mm_serve.cpp"), "-o", serve_obj], verbose)
+    # The mixed-length kernel is built once per workgroup size, each in an object of its own:
+    # the three device compiles run side by side.
+    mix_objs = [os.path.join(OUT_DIR, f"mm_mix{nt}.o") for nt in (512, 1024)]
+    _run_all([[HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, "mm_hip.hip"), "-o", hip_obj]] +
+             [[HIPCC, *HIP_FLAGS, *extra, f"-DMM_MIX_NT={nt}", "-c", os.path.join(CSRC, "mm_mix.hip"), "-o", o]
+              for nt, o in zip((512, 1024), mix_objs)] +
+             [["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{GMP_INCLUDE}", "-c",
+               os.path.join(CSRC, "mm_host.cpp"), "-o", host_obj],
+              ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{GMP_INCLUDE}", "-c",
+               os.path.join(CSRC, "mm_serve.cpp"), "-o", serve_obj]], verbose)
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, host_obj, serve_obj,
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, *mix_objs, host_obj, serve_obj,
           GMP_LIB, "-pthread"], verbose)
     os.replace(tmp, out)
     return out

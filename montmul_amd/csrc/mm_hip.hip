@@ -21,109 +21,30 @@
 #include <vector>
 
 #include "../../include/montmul_hip.h"
-#include "mm_step.h"
-#include "mm_v2_step.h"
+#include "mm_slot_steps.h"
 
 namespace mm {
 
-struct KArgs {
-    Tab t;
-    HalfTab ht;
-    Slots s;
-};
-
 // ---- Kernels ---------------------------------------------------------------------------------
 
-// Up to `bits` steps on slot first + blockIdx.x (reference: `steps`, kernels/src/lib.rs:883-927).
-// BARE runs the squaring alone and leaves x unchanged (lib.rs:949-961).
+// Up to `bits` steps on slot first + blockIdx.x.
 template <bool GLOBAL, bool WRAP, bool BARE>
 __global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_t bits) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const Slots& S = A.s;
-    const uint32_t slot = first + blockIdx.x;
-    const uint32_t n = A.t.n, k = S.k;
-    Work w;
-    if (GLOBAL) {
-        w.a1 = S.work1 + (size_t)slot * n;
-        w.a2 = S.work2 + (size_t)slot * n;
-        w.lt = S.lim + (size_t)slot * 3 * k;
-        w.ls = w.lt + 2 * k;
-        w.sc = (uint64_t*)lds;
-    } else {
-        w.a2 = (F2*)lds;
-        w.a1 = (F1*)(lds + 16 * (size_t)n);
-        w.lt = (uint32_t*)(lds + 24 * (size_t)n);
-        w.ls = w.lt + 2 * k;
-        w.sc = (uint64_t*)(lds + ((24 * (size_t)n + 12 * (size_t)k + 7) & ~(size_t)7));
-    }
-    uint32_t* x = S.x + (size_t)slot * k;
-    const uint32_t* ex = S.exp + (size_t)slot * S.ew;
-    const uint32_t nb = S.nbits[slot];
-    uint32_t pos = S.pos[slot];
-    uint32_t bad = 0;
-    for (uint32_t it = 0; it < bits && pos < nb; it++, pos++) {
-        const uint32_t bit = (ex[pos >> 5] >> (31 - (pos & 31))) & 1;
-        const uint32_t c = bit ? S.base : 1;
-        bad |= square(w, x, k, c, A.t) != 0;
-        if (!BARE)
-            bad |= redc<WRAP>(w, x, k, A.t, A.ht, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
-                              S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
-    }
-    if (tid() == 0) {
-        S.pos[slot] = pos;
-        if (bad) S.err[slot] = 1;
-    }
+    steps_v1<GLOBAL, WRAP, BARE>(A, first + blockIdx.x, bits, lds);
 }
 
-// The same steps on register-resident transforms (wraparound path, n = 512 .. 8,192): the
-// vector lives in registers, exchanges go through LDS; in global mode T and x stay in HBM and
-// the exchanges use an LDS tile of at most 2,048 elements. The kernels are held to 128 VGPRs
-// (four waves per SIMD), so two 512-thread workgroups share a CU wherever LDS allows.
-#ifndef MM_V2_WAVES_PER_EU
-#define MM_V2_WAVES_PER_EU 4
-#endif
+// The kernels are held to 128 VGPRs (four waves per SIMD), so two 512-thread workgroups share a
+// CU wherever LDS allows.
 template <uint32_t LOGN, bool GLOBAL, bool BARE>
 __global__ __launch_bounds__(LOGN > 12 ? 1024 : 512) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_PER_EU))) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const Slots& S = A.s;
-    const uint32_t slot = first + blockIdx.x;
-    const uint32_t n = A.t.P.n, k = S.k;
-    v2::Xbuf X;
-    uint32_t* Tl;
-    uint64_t* sc;
-    if (GLOBAL) {
-        // Transform vectors in registers between exchanges, exchanges through an LDS tile of
-        // at most 2,048 elements, T and x in HBM.
-        constexpr uint32_t tile = v2::Tiling<LOGN, true>::tile;
-        X.z2 = (v2::Z2*)lds;
-        X.z1 = (v2::Z1*)(lds + 16 * tile);
-        Tl = S.lim + (size_t)slot * 3 * k;
-        sc = (uint64_t*)(lds + 24 * tile);
-    } else {
-        X.z2 = (v2::Z2*)lds;
-        X.z1 = (v2::Z1*)(lds + 16 * (size_t)n);
-        Tl = (uint32_t*)(lds + 24 * (size_t)n);
-        sc = (uint64_t*)(lds + 24 * (size_t)n + 8 * (size_t)k);
-    }
-    uint32_t* x = S.x + (size_t)slot * k;
-    const uint32_t* ex = S.exp + (size_t)slot * S.ew;
-    const uint32_t nb = S.nbits[slot];
-    uint32_t pos = S.pos[slot];
-    uint32_t bad = 0;
-    for (uint32_t it = 0; it < bits && pos < nb; it++, pos++) {
-        __syncthreads();  // x written by every thread in the previous step
-        const uint32_t bit = (ex[pos >> 5] >> (31 - (pos & 31))) & 1;
-        const F1* iw1 = bit ? A.iwc1 : A.t.iw1;
-        const v2::S2* iw2 = bit ? A.iwc2 : A.t.iw2;
-        bad |= v2::v2_square<LOGN, GLOBAL>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv);
-        if (!BARE)
-            bad |= v2::v2_redc<LOGN, GLOBAL>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
-                               S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
-    }
-    if (tid() == 0) {
-        S.pos[slot] = pos;
-        if (bad) S.err[slot] = 1;
-    }
+    steps_v2<LOGN, GLOBAL, BARE>(A, first + blockIdx.x, bits, lds);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t count, uint32_t v) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < count) p[j] = v;
 }
 
 // The unweighting tables times c (a batch's base) for the register-transform step: a 1 bit's
@@ -523,6 +444,12 @@ struct Ctx {
     size_t max_lds = 0;
     size_t max_dyn_lds = 0;     // dynamic LDS every step kernel can take
     size_t max_dyn_lds_v2 = 0;  // the same for the register-transform kernels
+    size_t max_dyn_lds_mix[2] = {0, 0};  // the mixed-length kernel, 512 and 1,024 threads
+    // Mixed-length launches: head counter, length classes and work items in one device
+    // buffer (grown on demand), and the latest launch's milliseconds per step.
+    unsigned char* mix_buf = nullptr;
+    size_t mix_cap = 0;
+    double mix_ms_per_step = 0;
     std::string name;
     std::mutex mu;
     std::map<uint32_t, std::shared_ptr<DevTables>> tabs;
@@ -709,6 +636,8 @@ int set_lds_limit(Ctx* C) {
         }
         C->max_dyn_lds_v2 = std::min(C->max_dyn_lds_v2 ? C->max_dyn_lds_v2 : dyn, dyn);
     }
+    HIPCHK(mix_lds_limit<512>(C->max_lds, &C->max_dyn_lds_mix[0]));
+    HIPCHK(mix_lds_limit<1024>(C->max_lds, &C->max_dyn_lds_mix[1]));
     C->max_dyn_lds = dyn_max;
     return 0;
 }
@@ -858,6 +787,144 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
     return MM_OK;
 }
 
+// ---- Several lengths in one launch ----------------------------------------------------------
+// The length class of a batch in the mixed launch of nt threads: the register-transform step in
+// global mode (kind log2 n), or the radix-4 step, LDS-resident when its vector fits in 64 KiB.
+// Allocates the HBM buffers that mode needs and the batch's own plan did not. Writes the LDS
+// bytes a workgroup needs.
+int mix_class(State* S, uint32_t nt, MixClass* out, uint64_t* lds) {
+    const uint32_t n = S->n, k = S->k;
+    const size_t sl = S->slots;
+    memset(out, 0, sizeof(*out));
+    if (S->v2) {
+        if (!S->d_lim && dalloc(&S->d_lim, sl * 3 * k)) return 1;
+        out->kind = ilog2(n);
+        out->v = S->vargs();
+        *lds = 24ull * std::min<uint32_t>(n, n > 4096 ? 4096 : 2048) + lds_scratch_bytes(nt);  // v2::Tiling
+        return 0;
+    }
+    const uint64_t resident = lds_bytes_resident(n, k, nt);
+    const bool global = resident > 65536;
+    if (global && ((!S->d_work1 && dalloc(&S->d_work1, sl * n)) || (!S->d_work2 && dalloc(&S->d_work2, sl * n)) ||
+                   (!S->d_lim && dalloc(&S->d_lim, sl * 3 * k))))
+        return 1;
+    out->kind = MIX_V1 | (global ? MIX_V1_GLOBAL : 0) | (S->wrap ? MIX_V1_WRAP : 0);
+    out->k = S->args();
+    *lds = global ? lds_scratch_bytes(nt) : resident;
+    return 0;
+}
+
+// Up to `bits` steps on every slot with bits left in every batch of the group, in launches of
+// the mixed-length kernel. Returns MM_OK, the first slot whose carry-out check failed (its
+// batch's index in *which), or MM_EFAULT.
+int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which) {
+    if (count == 0) return MM_OK;
+    Ctx* C = G[0]->C;
+    for (size_t g = 0; g < count; g++)
+        if (G[g]->C != C) { fail("a group's batches must share one backend"); return MM_EFAULT; }
+    if (hipSetDevice(C->device) != hipSuccess) { fail("hipSetDevice failed"); return MM_EFAULT; }
+    uint64_t most = 0;
+    uint32_t nt = 512;
+    for (size_t g = 0; g < count; g++) {
+        const State* S = G[g];
+        for (uint32_t i = 0; i < S->slots; i++) most = std::max<uint64_t>(most, S->nbits[i] - S->pos[i]);
+        if (S->v2 && S->n / 8 > nt) nt = S->n / 8;
+        G[g]->last_ms = 0;
+        G[g]->last_launches = 0;
+    }
+    uint64_t left = std::min(bits, most);
+    if (left == 0) return MM_OK;
+    std::vector<MixClass> cls(count);
+    uint64_t lds = 0;
+    for (size_t g = 0; g < count; g++) {
+        uint64_t l = 0;
+        if (push(G[g]) || mix_class(G[g], nt, &cls[g], &l)) return MM_EFAULT;
+        lds = std::max(lds, l);
+    }
+    const size_t lim = C->max_dyn_lds_mix[nt == 1024];
+    if (lds > lim) {
+        fail("mixed launch needs " + std::to_string(lds) + " bytes of LDS, " + std::to_string(lim) + " available");
+        return MM_EFAULT;
+    }
+    // Workgroups resident per CU: four waves per SIMD of 128 VGPRs, so 1,024 threads take a CU
+    // and 512-thread workgroups pair up when their LDS allows.
+    const uint32_t per_cu = nt == 1024 ? 1 : (2 * lds <= C->max_lds ? 2 : 1);
+    hipStream_t sm = C->stream;
+    std::vector<MixItem> items;
+    std::vector<double> cost;
+    while (left > 0) {
+        uint64_t s = left;
+        if (C->opts.max_launch_ms > 0)
+            s = C->mix_ms_per_step > 0 ? (uint64_t)std::max(1.0, C->opts.max_launch_ms / C->mix_ms_per_step) : 16;
+        s = std::min<uint64_t>(std::min(s, left), 0xffffffffu);
+        // Work items: every slot with bits left, the most work first (longest-processing-time
+        // order: the longest chains start at once, short ones fill in behind them).
+        items.clear();
+        cost.clear();
+        for (size_t g = 0; g < count; g++) {
+            const State* S = G[g];
+            const double per_step = (double)S->n * ilog2(S->n) * (S->v2 ? 1.0 : 3.0);
+            for (uint32_t i = 0; i < S->slots; i++) {
+                const uint64_t r = std::min<uint64_t>(s, S->nbits[i] - S->pos[i]);
+                if (r == 0) continue;
+                items.push_back(MixItem{(uint32_t)g, i});
+                cost.push_back((double)r * per_step);
+            }
+        }
+        std::vector<uint32_t> order(items.size());
+        for (uint32_t i = 0; i < order.size(); i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+        std::vector<MixItem> queue(items.size());
+        for (size_t i = 0; i < order.size(); i++) queue[i] = items[order[i]];
+        const size_t cls_off = 256, items_off = cls_off + ((count * sizeof(MixClass) + 255) & ~(size_t)255);
+        const size_t need = items_off + queue.size() * sizeof(MixItem);
+        if (need > C->mix_cap) {
+            if (hipStreamSynchronize(sm) != hipSuccess) { fail("stream synchronize failed"); return MM_EFAULT; }
+            if (C->mix_buf) hipFree(C->mix_buf);
+            C->mix_buf = nullptr;
+            C->mix_cap = 0;
+            if (dalloc(&C->mix_buf, need * 2)) return MM_EFAULT;
+            C->mix_cap = need * 2;
+        }
+        uint32_t* head = (uint32_t*)C->mix_buf;
+        const MixClass* dcls = (const MixClass*)(C->mix_buf + cls_off);
+        const MixItem* ditems = (const MixItem*)(C->mix_buf + items_off);
+        hipError_t ce = hipMemcpyAsync(C->mix_buf + cls_off, cls.data(), count * sizeof(MixClass), hipMemcpyHostToDevice, sm);
+        if (ce == hipSuccess)
+            ce = hipMemcpyAsync(C->mix_buf + items_off, queue.data(), queue.size() * sizeof(MixItem), hipMemcpyHostToDevice, sm);
+        if (ce != hipSuccess) { hip_fail("mixed launch upload", ce); return MM_EFAULT; }
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sm, head, 1u, 0u);
+        const uint32_t grid = (uint32_t)std::min<size_t>(queue.size(), (size_t)C->cus * per_cu);
+        hipEventRecord(C->ev0, sm);
+        const hipError_t le = nt == 1024 ? mix_launch<1024>(grid, lds, sm, dcls, ditems, (uint32_t)queue.size(), head, (uint32_t)s)
+                                         : mix_launch<512>(grid, lds, sm, dcls, ditems, (uint32_t)queue.size(), head, (uint32_t)s);
+        if (le != hipSuccess) { hip_fail("mixed kernel launch", le); return MM_EFAULT; }
+        hipEventRecord(C->ev1, sm);
+        hipError_t se = hipEventSynchronize(C->ev1);
+        if (se != hipSuccess) { hip_fail("mixed kernel", se); return MM_EFAULT; }
+        float ms = 0;
+        hipEventElapsedTime(&ms, C->ev0, C->ev1);
+        C->mix_ms_per_step = std::max(1e-6, (double)ms) / (double)s;
+        int64_t first = MM_OK;
+        for (size_t g = 0; g < count; g++) {
+            State* S = G[g];
+            S->last_ms += ms;
+            S->last_launches++;
+            const int64_t r = check_err(S, "run");
+            if (r == MM_EFAULT) return r;
+            if (r != MM_OK && first == MM_OK) {
+                first = r;
+                if (which) *which = g;
+            }
+            for (uint32_t i = 0; i < S->slots; i++) S->pos[i] += (uint32_t)std::min<uint64_t>(s, S->nbits[i] - S->pos[i]);
+            S->x_host_valid = false;
+        }
+        if (first != MM_OK) return first;
+        left -= s;
+    }
+    return MM_OK;
+}
+
 }  // namespace
 
 // ---- C ABI -------------------------------------------------------------------------------------
@@ -897,6 +964,7 @@ int mm_hip_open(const mm_options* opts, mm_backend* out) {
     }
     out->self = C;
     out->set_modulus = mm_set_modulus;
+    out->run_group = mm_run_group;
     out->create = mm_create;
     out->destroy = mm_destroy;
     out->load = mm_load;
@@ -921,6 +989,7 @@ void mm_hip_close(mm_backend* be) {
     if (!C) return;
     hipSetDevice(C->device);
     C->tabs.clear();
+    if (C->mix_buf) hipFree(C->mix_buf);
     if (C->ev0) hipEventDestroy(C->ev0);
     if (C->ev1) hipEventDestroy(C->ev1);
     if (C->stream) hipStreamDestroy(C->stream);
@@ -1108,6 +1177,17 @@ int mm_set_modulus(void* st, size_t i, const uint32_t* nl, const uint32_t* npl, 
 
 int64_t mm_run(void* st, uint64_t bits) { return steps((State*)st, bits, false); }
 int64_t mm_run_bare(void* st, uint64_t bits) { return steps((State*)st, bits, true); }
+
+// Batches of several lengths stepped together by the mixed-length kernel (no reference
+// counterpart; see mm_backend.run_group).
+int64_t mm_run_group(void* const* states, size_t count, uint64_t bits, size_t* which) {
+    std::vector<State*> G(count);
+    for (size_t g = 0; g < count; g++) {
+        G[g] = (State*)states[g];
+        if (!G[g]) { fail("null batch in a group"); return MM_EFAULT; }
+    }
+    return group_steps(G.data(), count, bits, which);
+}
 
 int64_t mm_mul(void* st, size_t out, size_t a, size_t b) {  // gpu/mod.rs:386-391
     State* S = (State*)st;

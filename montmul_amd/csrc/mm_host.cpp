@@ -12,6 +12,8 @@
 #include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -307,6 +309,159 @@ int mm_modbatch_modulus(mm_modbatch* b, size_t i, uint32_t* out) {
     if (check_slot(b, i)) return 1;
     to_limbs(b->moduli[i].n.v, out, b->k);
     return 0;
+}
+
+// ---- Mixed lengths ----------------------------------------------------------------------------
+// A batch whose moduli need different transform lengths: one ModBatch per length (Params of
+// each modulus alone, batch.rs:20-25), stepped together by the backend's run_group. Slot i is
+// the i-th modulus given. No reference counterpart: ModBatch::new rejects mixed lengths
+// (batch.rs:64-77).
+struct mm_mixbatch {
+    std::vector<mm_modbatch*> groups;                  // ascending L
+    std::vector<std::vector<size_t>> members;          // group -> slots
+    std::vector<std::pair<uint32_t, uint32_t>> where;  // slot -> (group, index in the group)
+    ~mm_mixbatch() {
+        for (mm_modbatch* g : groups) delete g;
+    }
+};
+
+int mm_mixbatch_new(const mm_backend* be, size_t count, const uint32_t* limbs, const size_t* offsets, const int8_t* signs,
+                    uint32_t base, mm_mixbatch** out) {
+    *out = nullptr;
+    if (count == 0) return fail("empty batch");
+    std::map<uint32_t, std::vector<size_t>> by_len;
+    for (size_t i = 0; i < count; i++) {
+        Mpz n;
+        from_signed_limbs(n.v, signs[i], limbs + offsets[i], offsets[i + 1] - offsets[i]);
+        const size_t bits = significant_bits(n.v);
+        if (bits > 0xffffffffu) return fail("modulus too large");
+        uint32_t k, len;
+        if (mm_params_for_bits((uint32_t)bits, be->min_len, be->max_len, &k, &len)) return 1;
+        by_len[len].push_back(i);
+    }
+    std::unique_ptr<mm_mixbatch> M(new mm_mixbatch());
+    M->where.resize(count);
+    for (auto& kv : by_len) {
+        const std::vector<size_t>& idx = kv.second;
+        std::vector<uint32_t> l;
+        std::vector<size_t> off{0};
+        std::vector<int8_t> sg;
+        for (size_t i : idx) {
+            l.insert(l.end(), limbs + offsets[i], limbs + offsets[i + 1]);
+            off.push_back(l.size());
+            sg.push_back(signs[i]);
+        }
+        mm_modbatch* b;
+        if (mm_modbatch_new(be, idx.size(), l.data(), off.data(), sg.data(), base, &b)) return 1;
+        const uint32_t g = (uint32_t)M->groups.size();
+        M->groups.push_back(b);
+        M->members.push_back(idx);
+        for (size_t j = 0; j < idx.size(); j++) M->where[idx[j]] = {g, (uint32_t)j};
+    }
+    *out = M.release();
+    return 0;
+}
+
+void mm_mixbatch_free(mm_mixbatch* b) { delete b; }
+size_t mm_mixbatch_slots(const mm_mixbatch* b) { return b->where.size(); }
+size_t mm_mixbatch_groups(const mm_mixbatch* b) { return b->groups.size(); }
+
+static mm_modbatch* mix_slot(const mm_mixbatch* b, size_t i, size_t* j) {
+    if (i >= b->where.size()) {
+        fail("slot " + std::to_string(i) + " out of range");
+        return nullptr;
+    }
+    *j = b->where[i].second;
+    return b->groups[b->where[i].first];
+}
+
+int mm_mixbatch_params(const mm_mixbatch* b, size_t i, uint32_t* k, uint32_t* len) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    if (!g) return 1;
+    mm_modbatch_params(g, k, len);
+    return 0;
+}
+
+int mm_mixbatch_load(mm_mixbatch* b, size_t i, int sign, const uint32_t* limbs, size_t nl) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    return g ? mm_modbatch_load(g, j, sign, limbs, nl) : 1;
+}
+
+int mm_mixbatch_load_raw(mm_mixbatch* b, size_t i, int sign, const uint32_t* limbs, size_t nl) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    return g ? mm_modbatch_load_raw(g, j, sign, limbs, nl) : 1;
+}
+
+int mm_mixbatch_set_exponent(mm_mixbatch* b, size_t i, int sign, const uint32_t* limbs, size_t nl) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    return g ? mm_modbatch_set_exponent(g, j, sign, limbs, nl) : 1;
+}
+
+// Every group through one run_group call when the backend has it, else group after group.
+// Returns MM_OK, the first failing slot (its index in this batch), or MM_EFAULT.
+int64_t mm_mixbatch_run(mm_mixbatch* b, uint64_t bits) {
+    const mm_backend& be = b->groups[0]->be;
+    if (be.run_group && b->groups.size() > 1) {
+        std::vector<void*> st;
+        for (mm_modbatch* g : b->groups) st.push_back(g->state);
+        size_t which = 0;
+        const int64_t r = be.run_group(st.data(), st.size(), bits, &which);
+        if (r == MM_OK) return r;
+        if (r < 0 || which >= b->groups.size() || (size_t)r >= b->members[which].size()) {
+            g_err = be.last_error();
+            return r < 0 ? r : MM_EFAULT;
+        }
+        const size_t slot = b->members[which][(size_t)r];
+        g_err = std::string(be.last_error()) + " (slot " + std::to_string(slot) + " of the mixed batch)";
+        return (int64_t)slot;
+    }
+    for (size_t g = 0; g < b->groups.size(); g++) {
+        const int64_t r = mm_modbatch_run(b->groups[g], bits);
+        if (r == MM_OK) continue;
+        if (r < 0 || (size_t)r >= b->members[g].size()) return r < 0 ? r : MM_EFAULT;
+        return (int64_t)b->members[g][(size_t)r];
+    }
+    return MM_OK;
+}
+
+uint64_t mm_mixbatch_remaining(mm_mixbatch* b, size_t i) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    return g ? mm_modbatch_remaining(g, j) : 0;
+}
+
+int mm_mixbatch_read_raw(mm_mixbatch* b, size_t i, uint32_t* out) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    return g ? mm_modbatch_read_raw(g, j, out) : 1;
+}
+
+int mm_mixbatch_read(mm_mixbatch* b, size_t i, uint32_t* out) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    return g ? mm_modbatch_read(g, j, out) : 1;
+}
+
+int mm_mixbatch_is_one(mm_mixbatch* b, size_t i) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    return g ? mm_modbatch_is_one(g, j) : -1;
+}
+
+int mm_mixbatch_check_pow(mm_mixbatch* b, size_t i, uint32_t base, const uint32_t* e, size_t ne) {
+    size_t j;
+    mm_modbatch* g = mix_slot(b, i, &j);
+    return g ? mm_modbatch_check_pow(g, j, base, e, ne) : -1;
+}
+
+// The backend state of group `group` (groups in ascending L; NULL if out of range): for plan
+// queries and kernel timing.
+void* mm_mixbatch_group_state(mm_mixbatch* b, size_t group) {
+    return group < b->groups.size() ? b->groups[group]->state : nullptr;
 }
 
 // fermat (batch.rs:187-198).

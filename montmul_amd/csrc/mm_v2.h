@@ -502,11 +502,14 @@ template <int E, class Coef> __device__ __forceinline__ u64 carry_local(Coef coe
     }
     return (u64)acc;
 }
-template <int E> __device__ __forceinline__ u64 carry_fix(u32 t, u32* l, u64 hi, u64* sc, u64 cin) {
-    const u32 nt = blockDim.x, lane = t & 63, wave = t >> 6, nw = nt >> 6;
+// NT = 0: the carry spans all blockDim threads. NT > 0 (a multiple of 64): only threads t < NT
+// hold coefficients; the other waves join the barriers and write nothing.
+template <int E, u32 NT = 0> __device__ __forceinline__ u64 carry_fix(u32 t, u32* l, u64 hi, u64* sc, u64 cin) {
+    const u32 nt = NT ? NT : blockDim.x, lane = t & 63, wave = t >> 6, nw = nt >> 6;
+    const bool mine = NT == 0 || t < NT;
     const u64 up = __shfl_up(hi, 1, 64);
     __syncthreads();
-    if (lane == 63) sc[wave] = hi;
+    if (lane == 63 && mine) sc[wave] = hi;
     __syncthreads();
     u64 k = t == 0 ? cin : (lane == 0 ? sc[wave - 1] : up);
     bool ones = true;
@@ -523,7 +526,7 @@ template <int E> __device__ __forceinline__ u64 carry_fix(u32 t, u32* l, u64 hi,
     const u64 y = g | __ballot(live && ones);
     const u64 s0 = g + y;
     const bool wgen = s0 < g, wprop = y == ~0ull;
-    if (lane == 0) sc[nw + wave] = (u64)wgen | ((u64)wprop << 1);
+    if (lane == 0 && mine) sc[nw + wave] = (u64)wgen | ((u64)wprop << 1);
     __syncthreads();
     const u64 f = lane < nw ? sc[nw + lane] : 0;
     const u64 gw = __ballot(f & 1), yw = gw | __ballot((f & 2) != 0);
@@ -540,9 +543,9 @@ template <int E> __device__ __forceinline__ u64 carry_fix(u32 t, u32* l, u64 hi,
     __syncthreads();
     return sc[2 * nw];
 }
-template <int E, class Coef> __device__ __forceinline__ u64 carry_regs(u32 t, Coef coef, u32* l, u64* sc, u64 cin) {
+template <int E, u32 NT = 0, class Coef> __device__ __forceinline__ u64 carry_regs(u32 t, Coef coef, u32* l, u64* sc, u64 cin) {
     const u64 hi = carry_local<E>(coef, l);
-    return carry_fix<E>(t, l, hi, sc, cin);
+    return carry_fix<E, NT>(t, l, hi, sc, cin);
 }
 
 // Coefficient staging for the transposition: 3 words per coefficient in the exchange

@@ -29,24 +29,32 @@ __host__ __device__ inline uint64_t v2_lds_resident(uint32_t n, uint32_t k, uint
     return 24ull * n + 8ull * k + 8ull * (2 * (threads / 64) + 2);
 }
 
+// WIDE: the workgroup may have more threads than the vector's n / 8; threads t >= n / 8 join
+// every barrier and touch no memory (the mixed-length launch runs every length in workgroups
+// of the largest length's size). Without WIDE, blockDim = n / 8 and no thread is idle.
+#define MM_ACT(t) (!WIDE || (t) < T)
+
 // One step's squaring: T = c x^2 into Tl (2k limbs); returns nonzero on a carry out.
 // iw1, iw2: the unweighting, zeta^-j / n or, on a 1 bit, zeta^-j c / n.
-template <u32 LOGN, bool TILED>
+template <u32 LOGN, bool TILED, bool WIDE = false>
 __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, u64* sc, const u32* x, const F1* iw1,
                                          const S2* iw2, u32 inv) {
     typedef CPlan<LOGN> P;
     u32 t = opaque_tid();  // refreshed per phase: no address held across phases
     const u32 T = P::T, n = P::n;
+    constexpr u32 NT = WIDE ? P::T : 0;
     const u32 r0 = P::radix(0), ls0 = P::lstride(0);
     Z1 v1[8];
     Z2 v2[8];
+    if (MM_ACT(t)) {
 #pragma unroll
-    for (u32 e = 0; e < 8; e++) {
-        u32 pos = slot_pos(t, e, r0, ls0, T);
-        u32 xv = x[pos];
-        v1[e] = Op1::real_mul(xv, V.w1[pos]);
-        v2[e] = Op2::real_mul(xv, V.w2[pos]);
-        MM_GROUP_FENCE(e);
+        for (u32 e = 0; e < 8; e++) {
+            u32 pos = slot_pos(t, e, r0, ls0, T);
+            u32 xv = x[pos];
+            v1[e] = Op1::real_mul(xv, V.w1[pos]);
+            v2[e] = Op2::real_mul(xv, V.w2[pos]);
+            MM_GROUP_FENCE(e);
+        }
     }
     dif_regs<LOGN, TILED>(t, v1, v2, V.tw1, V.tw2, X);
 #pragma unroll
@@ -62,18 +70,20 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
     if constexpr (G::chunks == 1) {
         const Cbuf C = cbuf(X, n);
         __syncthreads();
+        if (MM_ACT(t)) {
 #pragma unroll
-        for (u32 e = 0; e < 8; e++) {
-            u32 pos = slot_pos(t, e, r0, ls0, T);
-            Z1 a = Op1::mul(v1[e], iw1[pos]);
-            Z2 b = Op2::mul(v2[e], iw2[pos]);
-            cput(C, pos, crt_lazy(a.re, b.re, inv));
-            cput(C, pos + n, crt_lazy(a.im, b.im, inv));
-            MM_GROUP_FENCE(e);
+            for (u32 e = 0; e < 8; e++) {
+                u32 pos = slot_pos(t, e, r0, ls0, T);
+                Z1 a = Op1::mul(v1[e], iw1[pos]);
+                Z2 b = Op2::mul(v2[e], iw2[pos]);
+                cput(C, pos, crt_lazy(a.re, b.re, inv));
+                cput(C, pos + n, crt_lazy(a.im, b.im, inv));
+                MM_GROUP_FENCE(e);
+            }
         }
         __syncthreads();
         t = opaque_tid();
-        top = carry_regs<16>(t, [&](int i) { return cget(C, 16 * t + i); }, l, sc, 0);
+        top = carry_regs<16, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 16 * t + i) : (u128)0; }, l, sc, 0);
     } else {
         // The tile holds 2 * tile = n coefficients: the real parts (coefficients 0 .. n-1) go
         // through it and are carried first, then the imaginary parts (n .. 2n-1) with the
@@ -83,43 +93,51 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         u32 im1[8];
         u64 im2[8];
         __syncthreads();
+        if (MM_ACT(t)) {
 #pragma unroll
-        for (u32 e = 0; e < 8; e++) {
-            u32 pos = slot_pos(t, e, r0, ls0, T);
-            Z1 a = Op1::mul(v1[e], iw1[pos]);
-            Z2 b = Op2::mul(v2[e], iw2[pos]);
-            cput(C, pos, crt_lazy(a.re, b.re, inv));
-            im1[e] = a.im;
-            im2[e] = b.im;
-            pin(im1[e]);
-            pin(im2[e]);
-            MM_GROUP_FENCE(e);
+            for (u32 e = 0; e < 8; e++) {
+                u32 pos = slot_pos(t, e, r0, ls0, T);
+                Z1 a = Op1::mul(v1[e], iw1[pos]);
+                Z2 b = Op2::mul(v2[e], iw2[pos]);
+                cput(C, pos, crt_lazy(a.re, b.re, inv));
+                im1[e] = a.im;
+                im2[e] = b.im;
+                pin(im1[e]);
+                pin(im2[e]);
+                MM_GROUP_FENCE(e);
+            }
         }
         __syncthreads();
         u32 lo[8];
         t = opaque_tid();
-        const u64 mid = carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i); }, lo, sc, 0);
+        const u64 mid = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) : (u128)0; }, lo, sc, 0);
+        if (MM_ACT(t)) {
 #pragma unroll
-        for (u32 i = 0; i < 8; i++) Tl[8 * t + i] = lo[i];
+            for (u32 i = 0; i < 8; i++) Tl[8 * t + i] = lo[i];
 #pragma unroll
-        for (u32 e = 0; e < 8; e++) cput(C, slot_pos(t, e, r0, ls0, T), crt_lazy(im1[e], im2[e], inv));
+            for (u32 e = 0; e < 8; e++) cput(C, slot_pos(t, e, r0, ls0, T), crt_lazy(im1[e], im2[e], inv));
+        }
         __syncthreads();
         u32 hi[8];
         t = opaque_tid();
-        const u64 top = carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i); }, hi, sc, mid);
+        const u64 top = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) : (u128)0; }, hi, sc, mid);
+        if (MM_ACT(t)) {
 #pragma unroll
-        for (u32 i = 0; i < 8; i++) Tl[n + 8 * t + i] = hi[i];
+            for (u32 i = 0; i < 8; i++) Tl[n + 8 * t + i] = hi[i];
+        }
         return top != 0;
     }
+    if (MM_ACT(t)) {
 #pragma unroll
-    for (u32 i = 0; i < 16; i++) Tl[16 * t + i] = l[i];
+        for (u32 i = 0; i < 16; i++) Tl[16 * t + i] = l[i];
+    }
     return top != 0;
 }
 
 // Reduction: x = T R^-1 (lazy) from T in Tl, through m = (T mod R) N' mod R and the cyclic
 // product m N mod (R - 1) (montmul/src/cpu.rs:452-484, 297-328). Returns nonzero on a carry
 // that must vanish but does not.
-template <u32 LOGN, bool TILED>
+template <u32 LOGN, bool TILED, bool WIDE = false>
 __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u64* sc, u32* x, const F1* np1, const F2* np2,
                                        const F1* ab1, const F2* ab2) {
     const VTab& V = A.t;
@@ -128,6 +146,7 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     typedef CPlan<LOGN - 1> PH;
     u32 t = opaque_tid();  // refreshed per phase: no address held across phases
     const u32 T = P::T, n = P::n, k = n, h = n / 2, Th = PH::T;
+    constexpr u32 NT = WIDE ? P::T : 0;
     const u32 r0 = P::radix(0), ls0 = P::lstride(0);
     const u32 inv = A.crt_inv;
     // Staging for k = n coefficients: the exchange buffer (one tile) holds 2 * tile of them.
@@ -136,44 +155,52 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     Z1 v1[8];
     Z2 v2[8];
     __syncthreads();  // T's limbs written by every thread
+    if (MM_ACT(t)) {
 #pragma unroll
-    for (u32 e = 0; e < 8; e++) {
-        u32 pos = slot_pos(t, e, r0, ls0, T);
-        u32 tv = Tl[pos];
-        v1[e] = Op1::real_mul(tv, V.w1[pos]);
-        v2[e] = Op2::real_mul(tv, V.w2[pos]);
-        MM_GROUP_FENCE(e);
+        for (u32 e = 0; e < 8; e++) {
+            u32 pos = slot_pos(t, e, r0, ls0, T);
+            u32 tv = Tl[pos];
+            v1[e] = Op1::real_mul(tv, V.w1[pos]);
+            v2[e] = Op2::real_mul(tv, V.w2[pos]);
+            MM_GROUP_FENCE(e);
+        }
     }
     dif_regs<LOGN, TILED>(t, v1, v2, V.tw1, V.tw2, X);
     t = opaque_tid();
+    if (MM_ACT(t)) {
 #pragma unroll
-    for (u32 e = 0; e < 8; e++) {
-        v1[e] = Op1::mul(v1[e], np1[8 * t + e]);
-        v2[e] = Op2::mul(v2[e], np2[8 * t + e]);
-        MM_GROUP_FENCE(e);
+        for (u32 e = 0; e < 8; e++) {
+            v1[e] = Op1::mul(v1[e], np1[8 * t + e]);
+            v2[e] = Op2::mul(v2[e], np2[8 * t + e]);
+            MM_GROUP_FENCE(e);
+        }
     }
     dit_regs<LOGN, TILED>(t, v1, v2, V.itw1, V.itw2, X);
     __syncthreads();
     t = opaque_tid();
+    if (MM_ACT(t)) {
 #pragma unroll
-    for (u32 e = 0; e < 8; e++) {
-        u32 pos = slot_pos(t, e, r0, ls0, T);
-        Z1 a = Op1::mul(v1[e], V.iw1[pos]);
-        Z2 b = Op2::mul(v2[e], V.iw2[pos]);
-        cput(C, pos, crt_lazy(a.re, b.re, inv));
-        MM_GROUP_FENCE(e);
+        for (u32 e = 0; e < 8; e++) {
+            u32 pos = slot_pos(t, e, r0, ls0, T);
+            Z1 a = Op1::mul(v1[e], V.iw1[pos]);
+            Z2 b = Op2::mul(v2[e], V.iw2[pos]);
+            cput(C, pos, crt_lazy(a.re, b.re, inv));
+            MM_GROUP_FENCE(e);
+        }
     }
     __syncthreads();
     t = opaque_tid();
     u32 m[8];
-    carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i); }, m, sc, 0);
+    carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) : (u128)0; }, m, sc, 0);
     t = opaque_tid();
     // m packed as z_j = m_2j + i m_2j+1 at half-length positions j = 4t .. 4t+3.
+    if (MM_ACT(t)) {
 #pragma unroll
-    for (u32 i = 0; i < 4; i++) {
-        u32 q = sw(4 * t + i);
-        X.z1[q] = Z1{mod1(m[2 * i]), mod1(m[2 * i + 1])};
-        X.z2[q] = Z2{m[2 * i], m[2 * i + 1]};
+        for (u32 i = 0; i < 4; i++) {
+            u32 q = sw(4 * t + i);
+            X.z1[q] = Z1{mod1(m[2 * i]), mod1(m[2 * i + 1])};
+            X.z2[q] = Z2{m[2 * i], m[2 * i + 1]};
+        }
     }
     __syncthreads();
     t = opaque_tid();
@@ -224,24 +251,28 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     t = opaque_tid();
     // G = (cyclic + T mod R) mod (R - 1), x = floor(T / R) + G.
     u32 xs[8];
-    u64 cc = carry_regs<8>(t, [&](int i) { return cget(C, 8 * t + i) + Tl[8 * t + i]; }, xs, sc, 0);
+    u64 cc = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) + Tl[8 * t + i] : (u128)0; }, xs, sc, 0);
 #pragma nounroll
     for (int guard = 0; cc != 0 && guard < 4; guard++) {
         u32 prev[8];
 #pragma unroll
         for (u32 i = 0; i < 8; i++) prev[i] = xs[i];
-        cc = carry_regs<8>(t, [&](int i) { return (u128)prev[i]; }, xs, sc, cc);
+        cc = carry_regs<8, NT>(t, [&](int i) { return (u128)prev[i]; }, xs, sc, cc);
     }
     t = opaque_tid();
     u32 lo[8];
 #pragma unroll
     for (u32 i = 0; i < 8; i++) lo[i] = xs[i];
-    u64 top = carry_regs<8>(t, [&](int i) { return (u128)lo[i] + Tl[k + 8 * t + i]; }, xs, sc, 0);
+    u64 top = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? (u128)lo[i] + Tl[k + 8 * t + i] : (u128)0; }, xs, sc, 0);
     t = opaque_tid();
+    if (MM_ACT(t)) {
 #pragma unroll
-    for (u32 i = 0; i < 8; i++) x[8 * t + i] = xs[i];
+        for (u32 i = 0; i < 8; i++) x[8 * t + i] = xs[i];
+    }
     return (top != 0) | (cc != 0);
 }
+
+#undef MM_ACT
 
 }  // namespace v2
 }  // namespace mm

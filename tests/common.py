@@ -8,7 +8,7 @@ methods); `ModBatch` and `fermat` are the host API under test.
 
 import random
 
-from montmul_amd import MontmulError, ModBatch, Params, fermat
+from montmul_amd import MixBatch, MontmulError, ModBatch, Params, fermat
 
 
 def rand_odd(rng, bits):
@@ -411,3 +411,50 @@ def serve_mix(server, make, moduli, base=3, timeout_s=120.0):
         alone.run(1 << 30)
         assert server.raw(i) == alone.read_raw(0), f"request {i}: raw differs alone and in the mix"
     return got
+
+
+def mixed_lengths(make, bit_sizes, ebits, chunk, seed=9):
+    """A MixBatch over moduli of the given sizes (several transform lengths) against plain
+    per-length ModBatches on the same backend: every slot loaded with 1 or a random x, random
+    exponents of 0 .. ebits bits (so slots finish at different steps), run(chunk) until no bits
+    remain. Every slot: read == x^(2^nbits) 3^e mod N, and raw limbs equal to its length's
+    ModBatch. No reference counterpart (ModBatch holds one length, batch.rs:64-77)."""
+    rng = random.Random(seed)
+    moduli = [rand_odd(rng, b) for b in bit_sizes]
+    xs, exps = [], []
+    for i, n in enumerate(moduli):
+        xs.append(1 if i % 3 else rng.randrange(n))
+        nb = (0, 1, ebits)[i % 3] if i < 3 else rng.randrange(ebits // 2, ebits + 1)
+        exps.append(rand_exponent(rng, nb) if nb else 0)
+    mix = MixBatch(make(), moduli, 3)
+    lengths = sorted({mix.params(i).len for i in range(len(moduli))})
+    assert mix.groups() == len(lengths)
+    for i in range(len(moduli)):
+        mix.load(i, xs[i])
+        mix.set_exponent(i, exps[i])
+    rounds = 0
+    while any(mix.remaining(i) for i in range(len(moduli))):
+        mix.run(chunk)
+        rounds += 1
+        assert rounds <= ebits // chunk + 2
+    for i, n in enumerate(moduli):
+        nb = exps[i].bit_length()
+        assert mix.read(i) == pow(xs[i], 1 << nb, n) * pow(3, exps[i], n) % n, f"slot {i}, {n.bit_length()} bits"
+    for length in lengths:
+        idx = [i for i in range(len(moduli)) if mix.params(i).len == length]
+        plain = ModBatch(make(), [moduli[i] for i in idx], 3)
+        for j, i in enumerate(idx):
+            plain.load(j, xs[i])
+            plain.set_exponent(j, exps[i])
+        plain.run(ebits)
+        for j, i in enumerate(idx):
+            assert plain.read_raw(j) == mix.read_raw(i), f"slot {i}, L = {length}"
+        plain.free()
+    with_err = False
+    try:
+        mix.read(len(moduli))
+    except MontmulError:
+        with_err = True
+    assert with_err
+    mix.free()
+    return lengths

@@ -1,0 +1,76 @@
+"""Mixed-length batches on the GPU: slots of several transform lengths stepped by the
+mixed-length kernel (one persistent launch per chunk, workgroups taking per-slot descriptors
+from a device queue). Every slot must equal pow(x, 2^nbits) 3^e mod N and carry the same raw
+limbs as its length's plain ModBatch (the per-length step kernels). No reference counterpart:
+the reference's ModBatch holds one length (batch.rs:64-77)."""
+
+import pytest
+
+import common
+from montmul_amd import Hip, MixBatch
+
+pytestmark = pytest.mark.gpu
+
+# Sizes per transform length, L = 8 .. 16,384 (register-transform lengths 1,024 .. 16,384).
+SMALL = [64, 200, 1000, 2000]                        # L = 8, 16, 64, 128
+V2 = [6000, 12000, 20000, 40000, 60000, 100000]      # L = 512 .. 8,192
+LARGEST = [150000, 200000]                           # L = 16,384: 1,024-thread workgroups
+
+
+def test_mixed_all_lengths_1024_threads(hip_global):
+    """A group with L = 16,384 runs 1,024-thread workgroups: every shorter register-transform
+    length runs with idle waves."""
+    lengths = common.mixed_lengths(lambda: hip_global, SMALL + V2 + LARGEST + [3000, 8000, 130000], 64, 25)
+    assert lengths[-1] == 16384 and 256 in lengths and 8192 in lengths
+
+
+def test_mixed_up_to_8192_512_threads(hip):
+    """Without L = 16,384 the workgroups have 512 threads; batches planned LDS-resident get
+    their HBM buffers for the mixed launch on first use."""
+    lengths = common.mixed_lengths(lambda: hip, SMALL + V2 + [9000, 70000], 48, 16, seed=10)
+    assert lengths[-1] == 8192
+
+
+def test_mixed_with_radix4_global_length(hip_global):
+    """L = 32,768 (above the register-transform range) runs the radix-4 step in global mode
+    inside the mixed launch."""
+    lengths = common.mixed_lengths(lambda: hip_global, [300000, 5000, 100000, 16000], 16, 7, seed=11)
+    assert lengths == [512, 1024, 8192, 32768]
+
+
+def test_mixed_without_wraparound():
+    """The reduction by the full product N-hat (no half path) in the mixed launch."""
+    h = Hip(wraparound=False)
+    try:
+        common.mixed_lengths(lambda: h, [700, 3000, 20000, 60000], 32, 11, seed=12)
+    finally:
+        h.close()
+
+
+def test_mixed_launch_splitting():
+    """max_launch_ms splits a run into several mixed launches (steps per launch from the
+    previous launch's time)."""
+    h = Hip(max_launch_ms=1)
+    try:
+        common.mixed_lengths(lambda: h, [5000, 20000, 100000, 200000], 80, 80, seed=13)
+    finally:
+        h.close()
+
+
+def test_mixed_fermat_planted(hip_global):
+    """Full Fermat tests of known primes and composites of different lengths in one batch."""
+    ms = [2**4253 - 1, 2**4423 - 1, 2**9689 - 1, common.tribonacci(4202)]
+    ms = [m for m in ms if m % 2 == 1]
+    comp = [m * 3 + 2 for m in ms]
+    comp = [c if c % 2 else c + 1 for c in comp]
+    moduli = ms + comp
+    b = MixBatch(hip_global, moduli, 3)
+    for i, n in enumerate(moduli):
+        b.load(i, 1)
+        b.set_exponent(i, n - 1)
+    while any(b.remaining(i) for i in range(len(moduli))):
+        b.run(4096)
+    for i, n in enumerate(moduli):
+        assert b.is_one(i) == (pow(3, n - 1, n) == 1), f"slot {i}"
+    assert all(b.is_one(i) for i in range(len(ms)))
+    b.free()

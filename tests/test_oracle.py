@@ -250,3 +250,9 @@ def test_set_modulus_oracle(wrap):
 
 def test_set_modulus_gmp_oracle():
     common.set_modulus_admission(lambda: Gmp(), 1000)
+
+
+def test_mixbatch_oracle():
+    """Mixed lengths on the Cpu restatement (no joint run: the groups run one after another)."""
+    lengths = common.mixed_lengths(lambda: Cpu(wraparound=True), [64, 200, 300, 700, 1500, 90, 640], 40, 9)
+    assert lengths == [8, 16, 32, 64, 128]
