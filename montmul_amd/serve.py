@@ -1,7 +1,7 @@
 """Serving benchmark: Fermat PRP requests arriving as a Poisson process, served over length pools.
 
     python -m montmul_amd.serve [--rates 1,2,5,10,20,50,100,200] [--duration 20] \
-        [--min-bits 4096] [--max-bits 262144] [--planted 0.1] [--seed 3]
+        [--min-bits 4096] [--max-bits 262144] [--planted 0.1] [--seed 3] [--burst N]
 
 Requests are base-3 Fermat tests (exponent N - 1) of random odd moduli whose bit sizes are
 log-uniform in [min-bits, max-bits]. About `planted` of them are known primes in that range:
@@ -14,6 +14,10 @@ For each rate, prints one JSON line with:
 - aggregate squarings/s over the rate's wall time;
 - the same requests run afterwards as plain per-length batches (one fermat() batch per L,
   lengths one after another), for comparison.
+
+With --burst N, N requests arrive at once instead (one line, "rate_per_s": null). Both forms then
+start with all the work in hand, so their squarings/s compare the throughput of continuous
+batching across lengths with that of per-length batches.
 """
 
 import argparse
@@ -64,13 +68,28 @@ def workload(rng, rate, duration, min_bits, max_bits, planted_frac, primes):
         out.append((t, n, False))
 
 
+def burst_workload(rng, count, min_bits, max_bits, planted_frac, primes):
+    """`count` requests arriving at time 0, drawn as in workload()."""
+    out = []
+    for _ in range(count):
+        if primes and rng.random() < planted_frac:
+            out.append((0.0, rng.choice(primes), True))
+            continue
+        bits = int(math.exp(rng.uniform(math.log(min_bits), math.log(max_bits))))
+        out.append((0.0, rng.getrandbits(bits) | (1 << (bits - 1)) | 1, False))
+    return out
+
+
 def pct(v, q):
     v = sorted(v)
     return v[min(len(v) - 1, int(q * (len(v) - 1) + 0.5))] if v else None
 
 
 def run_rate(args, rate, rng, primes):
-    reqs = workload(rng, rate, args.duration, args.min_bits, args.max_bits, args.planted, primes)
+    if rate is None:
+        reqs = burst_workload(rng, args.burst, args.min_bits, args.max_bits, args.planted, primes)
+    else:
+        reqs = workload(rng, rate, args.duration, args.min_bits, args.max_bits, args.planted, primes)
     srv = mm.Server(base=3, pool_slots=args.pool_slots, target_ms=args.target_ms, keep_raw=True,
                     device=args.device)
     ids, res = {}, {}
@@ -84,9 +103,13 @@ def run_rate(args, rate, rng, primes):
                 res[r["id"]] = r
         ids[srv.submit(n)] = (n, planted)
     deadline = time.perf_counter() + args.drain
+    shown = time.perf_counter()
     while len(res) < len(ids) and time.perf_counter() < deadline:
         for r in srv.poll(256, 0.1):
             res[r["id"]] = r
+        if time.perf_counter() - shown > 20:
+            shown = time.perf_counter()
+            print(f"rate {rate}: {len(res)} of {len(ids)} requests done", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     done = [r for r in res.values() if r["prp"] >= 0]
     lat = [r["done_s"] - r["submit_s"] for r in done]
@@ -94,7 +117,7 @@ def run_rate(args, rate, rng, primes):
     squarings = sum(r["bits"] for r in done)
     bad_planted = [i for i, (n, planted) in ids.items() if planted and i in res and res[i]["prp"] != 1]
     # alone vs mix: a few finished requests again in a batch of their own
-    sample = random.Random(rate).sample(sorted(res), min(args.check_alone, len(res)))
+    sample = random.Random(rate if rate is not None else args.seed).sample(sorted(res), min(args.check_alone, len(res)))
     hip = mm.Hip(device=args.device)
     mismatch = []
     for i in sample:
@@ -147,11 +170,13 @@ def main():
     ap.add_argument("--check-alone", type=int, default=4)
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--burst", type=int, default=0, help="N requests at once instead of the rate sweep")
     args = ap.parse_args()
     rng = random.Random(args.seed)
     primes = planted_primes(args.min_bits, args.max_bits)
     ok = True
-    for rate in [float(r) for r in args.rates.split(",") if r]:
+    rates = [None] if args.burst else [float(r) for r in args.rates.split(",") if r]
+    for rate in rates:
         line = run_rate(args, rate, rng, primes)
         print(json.dumps(line), flush=True)
         ok &= line["planted_not_prime"] == 0 and line["alone_mismatch"] == 0 and line["failed"] == 0
