@@ -735,6 +735,47 @@ int64_t check_err(State* S, const char* what) {
     return MM_OK;
 }
 
+// Runs `reps` launches of one chunk on the context's stream and waits for them; writes their
+// device time. Two or more repeats are captured once into a HIP graph and replayed back to back,
+// with no host round trip between them (reference: the per-launch loop of Gpu::steps,
+// gpu/mod.rs:113-158, synchronizes after every launch). `launch` enqueues kernels only (no
+// memset nodes), and returns hipSuccess or the launch error.
+template <class Launch> int run_chunks(Ctx* C, uint64_t reps, Launch&& launch, float* ms) {
+    hipStream_t sm = C->stream;
+    *ms = 0;
+    if (reps <= 1) {
+        hipEventRecord(C->ev0, sm);
+        hipError_t le = launch();
+        if (le != hipSuccess) return hip_fail("step kernel launch", le);
+        hipEventRecord(C->ev1, sm);
+    } else {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch();
+        hipError_t ce = hipStreamEndCapture(sm, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return hip_fail("step chunk capture", le != hipSuccess ? le : ce);
+        }
+        hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ie != hipSuccess) return hip_fail("step chunk graph", ie);
+        hipEventRecord(C->ev0, sm);
+        hipError_t ge_err = hipSuccess;
+        for (uint64_t r = 0; r < reps && ge_err == hipSuccess; r++) ge_err = hipGraphLaunch(ge, sm);
+        hipEventRecord(C->ev1, sm);
+        hipError_t se = hipEventSynchronize(C->ev1);
+        hipGraphExecDestroy(ge);
+        if (ge_err != hipSuccess) return hip_fail("step chunk graph launch", ge_err);
+        if (se != hipSuccess) return hip_fail("step kernel", se);
+    }
+    hipError_t se = hipEventSynchronize(C->ev1);
+    if (se != hipSuccess) return hip_fail("step kernel", se);
+    hipEventElapsedTime(ms, C->ev0, C->ev1);
+    return 0;
+}
+
 // Runs up to `bits` steps on every slot with bits left (reference: Gpu::steps, gpu/mod.rs:113-158).
 int64_t steps(State* S, uint64_t bits, bool bare) {
     Ctx* C = S->C;
@@ -763,26 +804,28 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
             s = std::min(s, left);
         }
         s = std::min<uint64_t>(s, 0xffffffffu);
-        hipEventRecord(C->ev0, C->stream);
-        if (kern2)
-            hipLaunchKernelGGL(kern2, dim3(S->slots), dim3(S->threads), S->lds, C->stream, A2, 0u, (uint32_t)s);
-        else
-            hipLaunchKernelGGL(kern, dim3(S->slots), dim3(S->threads), S->lds, C->stream, A, 0u, (uint32_t)s);
-        hipError_t le = hipGetLastError();
-        if (le != hipSuccess) { hip_fail("step kernel launch", le); return MM_EFAULT; }
-        hipEventRecord(C->ev1, C->stream);
-        hipError_t se = hipEventSynchronize(C->ev1);
-        if (se != hipSuccess) { hip_fail("step kernel", se); return MM_EFAULT; }
+        // Once the rate is known, the remaining whole chunks go as replays of one graph; a carry
+        // failure is then reported after them (the failing slot's error word stays set).
+        const uint64_t reps = C->opts.max_launch_ms > 0 && S->ms_per_slot_step[bare] > 0 ? std::max<uint64_t>(1, left / s) : 1;
         float ms = 0;
-        hipEventElapsedTime(&ms, C->ev0, C->ev1);
+        if (run_chunks(C, reps,
+                       [&]() {
+                           if (kern2)
+                               hipLaunchKernelGGL(kern2, dim3(S->slots), dim3(S->threads), S->lds, C->stream, A2, 0u, (uint32_t)s);
+                           else
+                               hipLaunchKernelGGL(kern, dim3(S->slots), dim3(S->threads), S->lds, C->stream, A, 0u, (uint32_t)s);
+                           return hipGetLastError();
+                       },
+                       &ms))
+            return MM_EFAULT;
         S->last_ms += ms;
-        S->last_launches++;
-        S->ms_per_slot_step[bare] = std::max(1e-6, (double)ms) / ((double)s * std::max<uint32_t>(1, S->slots));
+        S->last_launches += (uint32_t)reps;
+        S->ms_per_slot_step[bare] = std::max(1e-6, (double)ms) / ((double)(s * reps) * std::max<uint32_t>(1, S->slots));
         int64_t r = check_err(S, bare ? "run_bare" : "run");
-        for (uint32_t i = 0; i < S->slots; i++) S->pos[i] += (uint32_t)std::min<uint64_t>(s, S->nbits[i] - S->pos[i]);
+        for (uint32_t i = 0; i < S->slots; i++) S->pos[i] += (uint32_t)std::min<uint64_t>(s * reps, S->nbits[i] - S->pos[i]);
         if (!bare) S->x_host_valid = false;
         if (r != MM_OK) return r;
-        left -= s;
+        left -= s * reps;
     }
     return MM_OK;
 }
@@ -893,34 +936,40 @@ int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which)
         if (ce == hipSuccess)
             ce = hipMemcpyAsync(C->mix_buf + items_off, queue.data(), queue.size() * sizeof(MixItem), hipMemcpyHostToDevice, sm);
         if (ce != hipSuccess) { hip_fail("mixed launch upload", ce); return MM_EFAULT; }
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sm, head, 1u, 0u);
+        // The upload must land before a replayed graph reads it.
+        if (hipStreamSynchronize(sm) != hipSuccess) { fail("mixed launch upload failed"); return MM_EFAULT; }
         const uint32_t grid = (uint32_t)std::min<size_t>(queue.size(), (size_t)C->cus * per_cu);
-        hipEventRecord(C->ev0, sm);
-        const hipError_t le = nt == 1024 ? mix_launch<1024>(grid, lds, sm, dcls, ditems, (uint32_t)queue.size(), head, (uint32_t)s)
-                                         : mix_launch<512>(grid, lds, sm, dcls, ditems, (uint32_t)queue.size(), head, (uint32_t)s);
-        if (le != hipSuccess) { hip_fail("mixed kernel launch", le); return MM_EFAULT; }
-        hipEventRecord(C->ev1, sm);
-        hipError_t se = hipEventSynchronize(C->ev1);
-        if (se != hipSuccess) { hip_fail("mixed kernel", se); return MM_EFAULT; }
+        // Whole chunks after the first go as replays of one graph (queue reset + launch); slots
+        // that finish early take their descriptor and return at once.
+        const uint64_t reps = C->opts.max_launch_ms > 0 && C->mix_ms_per_step > 0 ? std::max<uint64_t>(1, left / s) : 1;
         float ms = 0;
-        hipEventElapsedTime(&ms, C->ev0, C->ev1);
-        C->mix_ms_per_step = std::max(1e-6, (double)ms) / (double)s;
+        if (run_chunks(C, reps,
+                       [&]() {
+                           hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sm, head, 1u, 0u);
+                           hipError_t fe = hipGetLastError();
+                           if (fe != hipSuccess) return fe;
+                           return nt == 1024 ? mix_launch<1024>(grid, lds, sm, dcls, ditems, (uint32_t)queue.size(), head, (uint32_t)s)
+                                             : mix_launch<512>(grid, lds, sm, dcls, ditems, (uint32_t)queue.size(), head, (uint32_t)s);
+                       },
+                       &ms))
+            return MM_EFAULT;
+        C->mix_ms_per_step = std::max(1e-6, (double)ms) / (double)(s * reps);
         int64_t first = MM_OK;
         for (size_t g = 0; g < count; g++) {
             State* S = G[g];
             S->last_ms += ms;
-            S->last_launches++;
+            S->last_launches += (uint32_t)reps;
             const int64_t r = check_err(S, "run");
             if (r == MM_EFAULT) return r;
             if (r != MM_OK && first == MM_OK) {
                 first = r;
                 if (which) *which = g;
             }
-            for (uint32_t i = 0; i < S->slots; i++) S->pos[i] += (uint32_t)std::min<uint64_t>(s, S->nbits[i] - S->pos[i]);
+            for (uint32_t i = 0; i < S->slots; i++) S->pos[i] += (uint32_t)std::min<uint64_t>(s * reps, S->nbits[i] - S->pos[i]);
             S->x_host_valid = false;
         }
         if (first != MM_OK) return first;
-        left -= s;
+        left -= s * reps;
     }
     return MM_OK;
 }

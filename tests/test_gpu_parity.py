@@ -183,6 +183,38 @@ def test_long_run_split_into_launches():
         h.close()
 
 
+def test_graph_replayed_chunks_match_eager():
+    """With max_launch_ms, the whole chunks of a run after the first are replays of one captured
+    graph. Several runs in a row, with host reads and eager runs of a second backend between
+    them: every slot's raw limbs equal the single-launch (eager) backend's after every run."""
+    rng = random.Random(21)
+    split, eager = Hip(max_launch_ms=1), Hip()
+    try:
+        for bits in (20_000, 100_000):
+            moduli = [common.rand_odd(rng, bits) for _ in range(16)]
+            exps = [rng.getrandbits(700) | (1 << 699) for _ in range(16)]
+            a, b = ModBatch(split, moduli, 3), ModBatch(eager, moduli, 3)
+            for bt in (a, b):
+                for i, e in enumerate(exps):
+                    bt.load(i, 1)
+                    bt.set_exponent(i, e)
+            launches = 0
+            for chunk in (150, 250, 300):
+                a.run(chunk)
+                launches = max(launches, Hip.last_run_kernel_ms(a)[1])
+                b.run(chunk)
+                for i in range(16):
+                    assert a.read_raw(i) == b.read_raw(i), f"{bits} bits, slot {i}, after run({chunk})"
+            assert launches > 2  # chunks replayed as a graph
+            for i, n in enumerate(moduli):
+                assert a.remaining(i) == 0 and a.read(i) == pow(3, exps[i], n)
+            a.free()
+            b.free()
+    finally:
+        split.close()
+        eager.close()
+
+
 def test_two_batches_interleaved(hip):
     """Two live batches with different L, run alternately (tests/gpu.rs:227-258)."""
     rng = random.Random(13)
