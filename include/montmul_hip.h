@@ -304,6 +304,9 @@ typedef struct mm_server_options {
     double target_ms;     /* wall time of one pool round; steps per round follow from it */
     uint32_t max_chunk;   /* most steps per round (0: 4096) */
     int32_t keep_raw;     /* keep each result's final raw limbs for mm_server_raw */
+    uint64_t hbm_bytes;   /* device memory all pools together may take (0: no limit). A pool's
+                           * slot count is cut to what fits; a length that does not fit waits,
+                           * and idle pools give their memory back while a length is waiting */
 } mm_server_options;
 
 typedef struct mm_server_result {
@@ -318,7 +321,14 @@ typedef struct mm_server_stats {
     uint64_t submitted, admitted, completed, failed, rounds, squarings, waiting;
     double busy_ms;       /* wall time of the pools' runs, summed over pools */
     uint32_t pools;
+    uint64_t hbm_used;    /* device memory the live pools hold, by mm_server_slot_bytes */
+    uint64_t releases;    /* idle pools that gave their memory back to a waiting length */
 } mm_server_stats;
+
+/* Device memory of one pool slot at transform length L in the server's accounting: on the HIP
+ * backend in global mode, x and the exponent (k words each), N'-hat and A|B (24 L bytes), the
+ * work vectors (12 L) and the limb staging (6 L), with k = L/2: 46 L + 16 bytes. */
+uint64_t mm_server_slot_bytes(uint32_t len);
 
 int mm_server_new(const mm_server_options *opts, mm_open_fn open, mm_close_fn close, void *user, mm_server **out);
 /* A server whose pools are HIP backends opened with `hip` (one context and stream per pool). */

@@ -84,7 +84,7 @@ class Options(C.Structure):
 class ServerOptions(C.Structure):
     """`mm_server_options`."""
     _fields_ = [("base", C.c_uint32), ("pool_slots", C.c_uint32), ("target_ms", C.c_double),
-                ("max_chunk", C.c_uint32), ("keep_raw", C.c_int32)]
+                ("max_chunk", C.c_uint32), ("keep_raw", C.c_int32), ("hbm_bytes", C.c_uint64)]
 
 
 class ServerResult(C.Structure):
@@ -96,7 +96,8 @@ class ServerResult(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
                 ("failed", C.c_uint64), ("rounds", C.c_uint64), ("squarings", C.c_uint64),
-                ("waiting", C.c_uint64), ("busy_ms", C.c_double), ("pools", C.c_uint32)]
+                ("waiting", C.c_uint64), ("busy_ms", C.c_double), ("pools", C.c_uint32),
+                ("hbm_used", C.c_uint64), ("releases", C.c_uint64)]
 
 
 class PlanInfo(C.Structure):
@@ -155,6 +156,7 @@ def _load_lib():
                                 C.POINTER(C.c_int8), C.c_uint32, C.c_uint64, C.POINTER(C.c_uint8)]),
         "mm_score_workload": (C.c_int, [C.c_uint32, C.c_size_t, C.c_uint32, C.c_uint64, u32p, u32p]),
         "mm_modbatch_check_pow": (C.c_int, [vp, C.c_size_t, C.c_uint32, u32p, C.c_size_t]),
+        "mm_server_slot_bytes": (C.c_uint64, [C.c_uint32]),
         "mm_run_group": (C.c_int64, [C.POINTER(vp), C.c_size_t, C.c_uint64, C.POINTER(C.c_size_t)]),
         "mm_mixbatch_new": (C.c_int, [C.POINTER(BackendTable), C.c_size_t, u32p, C.POINTER(C.c_size_t),
                                       C.POINTER(C.c_int8), C.c_uint32, C.POINTER(vp)]),
@@ -714,8 +716,10 @@ class Server:
     callable returning a backend such as the oracle's) replaces it, one backend per pool."""
 
     def __init__(self, base=3, pool_slots=256, target_ms=20.0, max_chunk=0, keep_raw=False, device=0,
-                 force_global=False, backend_factory=None):
-        self.options = ServerOptions(base, pool_slots, target_ms, max_chunk, int(bool(keep_raw)))
+                 force_global=False, backend_factory=None, hbm_bytes=0):
+        """hbm_bytes: device memory all pools together may take (0: no limit), counted with
+        slot_bytes(L) per slot; a length that finds no room waits for an idle pool's memory."""
+        self.options = ServerOptions(base, pool_slots, target_ms, max_chunk, int(bool(keep_raw)), hbm_bytes)
         self._h = C.c_void_p()
         self._keep = []
         if backend_factory is None:
@@ -738,6 +742,11 @@ class Server:
             r = lib.mm_server_new(C.byref(self.options), self._cb[0], self._cb[1], None, C.byref(self._h))
         if r != 0:
             raise MontmulError(_host_error() or (lib.mm_server_last_error(None) or b"").decode())
+
+    @staticmethod
+    def slot_bytes(length):
+        """Device memory of one pool slot at transform length L in the server's accounting."""
+        return lib.mm_server_slot_bytes(length)
 
     def submit(self, n):
         buf = to_limbs(int(n))

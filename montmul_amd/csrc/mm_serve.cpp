@@ -10,6 +10,11 @@
 //   3. retire finished slots: is_one on the slot's raw value (a read of that slot's k limbs
 //      only) and a completion record.
 // Pools run concurrently, so requests of different lengths share the GPU through streams.
+//
+// Admission by device memory: with hbm_bytes set, a pool reserves slots x mm_server_slot_bytes(L)
+// from the budget when it creates its batch, cut to what is left. A length that finds no room
+// waits; a pool with no active slot and no waiting request frees its batch while another length
+// waits, and waiting pools retry on every release.
 
 #include <gmp.h>
 
@@ -67,6 +72,9 @@ struct mm_server {
     mm_server_stats stats{};
     std::string last_error_text;
     void* owned_user = nullptr;  // freed with the server (the HIP factory's options)
+    uint64_t hbm_used = 0;       // bytes reserved by live pool batches
+    uint64_t hbm_epoch = 0;      // counts releases, so that waiting pools retry
+    uint32_t deferred = 0;       // pools waiting for device memory
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -89,6 +97,9 @@ struct Pool {
     std::vector<uint64_t> slot_bits;
     std::vector<uint32_t> free_slots;
     double ms_per_step = 0;
+    uint64_t bytes = 0;          // reserved for this pool's batch (guarded by S->mu)
+    bool deferred = false;       // waiting for device memory (guarded by S->mu)
+    uint64_t seen_epoch = 0;     // S->hbm_epoch at the last failed reservation
 
     void finish(size_t slot, int prp, const char* err) {
         mm_server_result r{};
@@ -116,14 +127,43 @@ struct Pool {
         free_slots.push_back((uint32_t)slot);
     }
 
-    // Creates the batch on first use, every slot on the first request's modulus.
+    // Creates the batch on first use, every slot on the first request's modulus. Returns 0, 1 on
+    // error, or 2 when the memory budget has no room for a slot yet.
     int ensure_batch(const Request& r) {
         if (batch) return 0;
         if (!be_open) {
             if (S->open(S->user, &be)) return fail("cannot open a backend for length " + std::to_string(len));
             be_open = true;
         }
-        const uint32_t cnt = std::max<uint32_t>(1, S->opt.pool_slots);
+        uint32_t cnt = std::max<uint32_t>(1, S->opt.pool_slots);
+        const uint64_t per = mm_server_slot_bytes(len);
+        {
+            std::lock_guard<std::mutex> g(S->mu);
+            if (S->opt.hbm_bytes) {
+                if (per > S->opt.hbm_bytes)
+                    return fail("a slot of length " + std::to_string(len) + " needs " + std::to_string(per) +
+                                " bytes, more than the budget of " + std::to_string(S->opt.hbm_bytes));
+                const uint64_t room = S->opt.hbm_bytes > S->hbm_used ? S->opt.hbm_bytes - S->hbm_used : 0;
+                cnt = (uint32_t)std::min<uint64_t>(cnt, room / per);
+                if (cnt == 0) {
+                    if (!deferred) {
+                        deferred = true;
+                        S->deferred++;
+                        // idle pools check S->deferred when woken
+                        for (auto& p : S->pools)
+                            if (p.second.get() != this) p.second->cv.notify_all();
+                    }
+                    seen_epoch = S->hbm_epoch;
+                    return 2;
+                }
+            }
+            bytes = (uint64_t)cnt * per;
+            S->hbm_used += bytes;
+            if (deferred) {
+                deferred = false;
+                S->deferred--;
+            }
+        }
         std::vector<uint32_t> limbs;
         std::vector<size_t> offsets(1, 0);
         std::vector<int8_t> signs(cnt, 1);
@@ -133,7 +173,9 @@ struct Pool {
         }
         if (mm_modbatch_new(&be, cnt, limbs.data(), offsets.data(), signs.data(), S->opt.base, &batch)) {
             batch = nullptr;
-            return fail(mm_host_last_error());
+            const std::string why = mm_host_last_error();
+            give_back();
+            return fail(why);
         }
         slot_req.assign(cnt, -1);
         slot_submit.assign(cnt, 0);
@@ -144,14 +186,41 @@ struct Pool {
         return 0;
     }
 
-    bool admit(Request& r) {
-        if (ensure_batch(r)) return false;
+    // Returns this pool's reservation to the budget and wakes every pool (guarded by S->mu
+    // inside).
+    void give_back() {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->hbm_used -= bytes;
+        bytes = 0;
+        S->hbm_epoch++;
+        for (auto& p : S->pools) p.second->cv.notify_all();
+    }
+
+    // Frees an idle batch so that a waiting length can use its memory.
+    void release() {
+        mm_modbatch_free(batch);
+        batch = nullptr;
+        slot_req.clear();
+        free_slots.clear();
+        {
+            std::lock_guard<std::mutex> g(S->mu);
+            S->stats.releases++;
+        }
+        give_back();
+    }
+
+    // 0: admitted; 1: failed; 2: no memory for this length's batch yet; 3: no free slot (the
+    // batch was cut to fewer slots than the requests taken).
+    int admit(Request& r) {
+        const int e = ensure_batch(r);
+        if (e) return e;
+        if (free_slots.empty()) return 3;
         const uint32_t slot = free_slots.back();
         const uint32_t one = 1;
         if (mm_modbatch_set_modulus(batch, slot, 1, r.n.data(), r.n.size()) ||
             mm_modbatch_load(batch, slot, 1, &one, 1) ||
             mm_modbatch_set_exponent(batch, slot, 1, r.e.data(), r.e.size()))
-            return false;
+            return 1;
         free_slots.pop_back();
         slot_req[slot] = r.id;
         slot_submit[slot] = r.t_submit;
@@ -159,7 +228,7 @@ struct Pool {
         std::lock_guard<std::mutex> g(S->mu);
         slot_admit[slot] = S->now();
         S->stats.admitted++;
-        return true;
+        return 0;
     }
 
     void reject(const Request& r, const std::string& why) {
@@ -183,18 +252,39 @@ struct Pool {
     void loop() {
         for (;;) {
             std::vector<Request> take;
+            bool idle_release = false;
             {
                 std::unique_lock<std::mutex> lk(S->mu);
-                cv.wait(lk, [&] { return S->stop || (!waiting.empty() && (!batch || !free_slots.empty())) || active() > 0; });
+                cv.wait(lk, [&] {
+                    return S->stop || (!waiting.empty() && batch && !free_slots.empty()) ||
+                           (!waiting.empty() && !batch && (!deferred || S->hbm_epoch != seen_epoch)) || active() > 0 ||
+                           (batch && active() == 0 && waiting.empty() && S->deferred > 0);
+                });
                 if (S->stop) break;
-                size_t room = batch ? free_slots.size() : std::max<uint32_t>(1, S->opt.pool_slots);
-                while (!waiting.empty() && take.size() < room) {
-                    take.push_back(std::move(waiting.front()));
-                    waiting.pop_front();
+                if (batch && active() == 0 && waiting.empty() && S->deferred > 0) {
+                    idle_release = true;
+                } else {
+                    size_t room = batch ? free_slots.size() : std::max<uint32_t>(1, S->opt.pool_slots);
+                    while (!waiting.empty() && take.size() < room) {
+                        take.push_back(std::move(waiting.front()));
+                        waiting.pop_front();
+                    }
                 }
             }
-            for (Request& r : take)
-                if (!admit(r)) reject(r, g_err.empty() ? std::string(mm_host_last_error()) : g_err);
+            if (idle_release) {
+                release();
+                continue;
+            }
+            for (size_t i = 0; i < take.size(); i++) {
+                const int a = admit(take[i]);
+                if (a == 2 || a == 3) {
+                    // No memory for this length yet, or no slot left: the rest wait, in order.
+                    std::lock_guard<std::mutex> g(S->mu);
+                    for (size_t j = take.size(); j-- > i;) waiting.push_front(std::move(take[j]));
+                    break;
+                }
+                if (a == 1) reject(take[i], g_err.empty() ? std::string(mm_host_last_error()) : g_err);
+            }
             if (!batch || active() == 0) continue;
             // One chunk of steps, sized from the measured time per step.
             uint64_t chunk = S->opt.max_chunk ? S->opt.max_chunk : 4096;
@@ -230,6 +320,15 @@ struct Pool {
         }
         if (batch) mm_modbatch_free(batch);
         batch = nullptr;
+        {
+            std::lock_guard<std::mutex> g(S->mu);
+            S->hbm_used -= bytes;
+            bytes = 0;
+            if (deferred) {
+                deferred = false;
+                S->deferred--;
+            }
+        }
         if (be_open && S->close) S->close(S->user, &be);
         be_open = false;
     }
@@ -248,6 +347,8 @@ void hip_close(void* user, mm_backend* be) {
 }  // namespace
 
 extern "C" {
+
+uint64_t mm_server_slot_bytes(uint32_t len) { return 46ull * len + 16; }
 
 int mm_server_new(const mm_server_options* o, mm_open_fn open, mm_close_fn close, void* user, mm_server** out) {
     *out = nullptr;
@@ -350,6 +451,7 @@ int mm_server_stats_get(mm_server* S, mm_server_stats* out) {
     std::lock_guard<std::mutex> g(S->mu);
     *out = S->stats;
     out->pools = (uint32_t)S->pools.size();
+    out->hbm_used = S->hbm_used;
     out->waiting = 0;
     for (auto& p : S->pools) out->waiting += p.second->waiting.size();
     return 0;

@@ -91,7 +91,7 @@ def run_rate(args, rate, rng, primes):
     else:
         reqs = workload(rng, rate, args.duration, args.min_bits, args.max_bits, args.planted, primes)
     srv = mm.Server(base=3, pool_slots=args.pool_slots, target_ms=args.target_ms, keep_raw=True,
-                    device=args.device)
+                    device=args.device, hbm_bytes=int(args.hbm_gib * (1 << 30)))
     ids, res = {}, {}
     t0 = time.perf_counter()
     for at, n, planted in reqs:
@@ -138,7 +138,8 @@ def run_rate(args, rate, rng, primes):
         "latency_p50_s": pct(lat, 0.5), "latency_p99_s": pct(lat, 0.99),
         "queue_p50_s": pct(queue, 0.5), "queue_p99_s": pct(queue, 0.99),
         "squarings": squarings, "wall_s": round(wall, 3), "sq_per_s": round(squarings / wall, 1),
-        "pools": st["pools"], "rounds": st["rounds"], "planted": sum(1 for v in ids.values() if v[1]),
+        "pools": st["pools"], "rounds": st["rounds"], "pool_releases": st["releases"],
+        "planted": sum(1 for v in ids.values() if v[1]),
         "planted_not_prime": len(bad_planted), "alone_checked": len(sample), "alone_mismatch": len(mismatch),
     }
     if not args.no_baseline and done:
@@ -171,6 +172,8 @@ def main():
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--no-baseline", action="store_true")
     ap.add_argument("--burst", type=int, default=0, help="N requests at once instead of the rate sweep")
+    ap.add_argument("--hbm-gib", type=float, default=0.0,
+                    help="device memory the pools may take, GiB (0: no limit; admission by footprint)")
     args = ap.parse_args()
     rng = random.Random(args.seed)
     primes = planted_primes(args.min_bits, args.max_bits)

@@ -118,6 +118,43 @@ def test_server_on_oracle():
         srv.close()
 
 
+def test_server_memory_budget_on_oracle():
+    """Admission by device memory: a budget with room for one slot of the longer length only.
+    Pools are cut to what fits, the other length waits, and an idle pool frees its batch for it;
+    every result is still right, and the reserved bytes never exceed the budget at the end."""
+    import random
+
+    from oracle_backends import Cpu
+
+    rng = random.Random(14)
+    long_len, short_len = 256, 64
+    budget = mm.Server.slot_bytes(long_len) + mm.Server.slot_bytes(short_len) // 2
+    moduli = [common.rand_odd(rng, 3000) for _ in range(3)] + [common.rand_odd(rng, 1000) for _ in range(4)]
+    moduli += [2**607 - 1, 2**1279 - 1]
+    srv = mm.Server(base=3, pool_slots=2, target_ms=5, keep_raw=True, hbm_bytes=budget,
+                    backend_factory=lambda: Cpu(wraparound=True))
+    try:
+        common.serve_mix(srv, lambda: Cpu(wraparound=True), moduli)
+        st = srv.stats()
+        assert st["completed"] == len(moduli) and st["failed"] == 0
+        assert st["releases"] >= 1 and st["hbm_used"] <= budget
+    finally:
+        srv.close()
+    # a length whose single slot exceeds the budget is refused, not queued forever
+    srv = mm.Server(base=3, pool_slots=2, hbm_bytes=mm.Server.slot_bytes(short_len),
+                    backend_factory=lambda: Cpu(wraparound=True))
+    try:
+        i = srv.submit(moduli[0])
+        got = []
+        for _ in range(100):
+            got += srv.poll(8, 0.1)
+            if got:
+                break
+        assert got and got[0]["id"] == i and got[0]["prp"] == -1
+    finally:
+        srv.close()
+
+
 def test_cli_arguments():
     """The command line's option rules (main.rs:32-66): unknown, repeated and value-less keys
     are errors; values parse as numbers where needed."""
