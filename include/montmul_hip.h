@@ -103,6 +103,10 @@ typedef struct mm_options {
                               * rule, plan.rs:26-42); by default the register-transform kernel
                               * (wraparound, L = 1,024 .. 8,192) runs in global mode, whose smaller
                               * LDS footprint fits more workgroups per CU and is faster there */
+    int32_t autotune;        /* time every plan that can run a new batch's length (radix-8
+                              * register kernel or radix-4 kernel, global mode with LDS tiles or
+                              * LDS-resident) for 16 steps on the batch's own slots and keep the
+                              * fastest; the choice is cached per length and slot-count octave */
 } mm_options;
 
 /* Launch plan of a batch. Replaces `LaunchPlan`/`Mode` (gpu/plan.rs:5-22). */
@@ -111,6 +115,8 @@ typedef struct mm_plan_info {
     uint32_t threads;        /* threads per workgroup */
     uint32_t lds_bytes;      /* dynamic LDS per workgroup */
     uint32_t slots_per_wg;   /* slots handled by one workgroup */
+    uint32_t kernel;         /* 4: radix-4 step (LDS or global); 8: register-resident radix-8 step */
+    int32_t tuned;           /* 1 when the autotuner chose this plan */
 } mm_plan_info;
 
 /* Replaces Gpu::device_count (gpu/mod.rs:70-72). Returns -1 on error. */

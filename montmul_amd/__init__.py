@@ -78,7 +78,7 @@ _CLOSE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(BackendTable))
 
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("force_global", C.c_int32), ("wraparound", C.c_int32),
-                ("max_launch_ms", C.c_uint32), ("prefer_lds", C.c_int32)]
+                ("max_launch_ms", C.c_uint32), ("prefer_lds", C.c_int32), ("autotune", C.c_int32)]
 
 
 class ServerOptions(C.Structure):
@@ -102,7 +102,7 @@ class ServerStats(C.Structure):
 
 class PlanInfo(C.Structure):
     _fields_ = [("global_", C.c_int32), ("threads", C.c_uint32), ("lds_bytes", C.c_uint32),
-                ("slots_per_wg", C.c_uint32)]
+                ("slots_per_wg", C.c_uint32), ("kernel", C.c_uint32), ("tuned", C.c_int32)]
 
 
 def _load_lib():
@@ -267,9 +267,12 @@ class Backend:
 class Hip(Backend):
     """The MI355X backend (replaces `Gpu`/`GpuOptions`, gpu/mod.rs:28-111)."""
 
-    def __init__(self, device=0, force_global=False, wraparound=True, max_launch_ms=0, prefer_lds=False):
+    def __init__(self, device=0, force_global=False, wraparound=True, max_launch_ms=0, prefer_lds=False,
+                 autotune=False):
+        """autotune: time every plan that fits each new batch's length (16 steps on its own
+        slots) and keep the fastest, cached per length and slot-count octave."""
         self.options = Options(device, int(bool(force_global)), int(bool(wraparound)), max_launch_ms,
-                               int(bool(prefer_lds)))
+                               int(bool(prefer_lds)), int(bool(autotune)))
         self.table = BackendTable()
         if lib.mm_hip_open(C.byref(self.options), C.byref(self.table)) != 0:
             raise MontmulError((lib.mm_last_error() or b"").decode())
@@ -313,7 +316,7 @@ class Hip(Backend):
         p = PlanInfo()
         lib.mm_plan(batch.state, C.byref(p))
         return {"global": bool(p.global_), "threads": p.threads, "lds_bytes": p.lds_bytes,
-                "slots_per_wg": p.slots_per_wg}
+                "slots_per_wg": p.slots_per_wg, "kernel": p.kernel, "tuned": bool(p.tuned)}
 
     @staticmethod
     def run_bare(batch, bits):
@@ -723,7 +726,7 @@ class Server:
         self._h = C.c_void_p()
         self._keep = []
         if backend_factory is None:
-            hip = Options(device, int(bool(force_global)), 1, 0, 0)
+            hip = Options(device, int(bool(force_global)), 1, 0, 0, 0)
             r = lib.mm_server_new_hip(C.byref(self.options), C.byref(hip), C.byref(self._h))
         else:
             def _open(user, out):

@@ -17,6 +17,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -450,6 +451,8 @@ struct Ctx {
     unsigned char* mix_buf = nullptr;
     size_t mix_cap = 0;
     double mix_ms_per_step = 0;
+    // Autotuner choices: (n, wraparound, slot-count octave) -> (register kernel, global mode).
+    std::map<std::tuple<uint32_t, bool, uint32_t>, std::pair<bool, bool>> tuned;
     std::string name;
     std::mutex mu;
     std::map<uint32_t, std::shared_ptr<DevTables>> tabs;
@@ -538,6 +541,7 @@ struct State {
     std::vector<uint32_t> e_dirty_list;
     bool exp_all = true;              // the device exponent buffer needs a full upload
     double ms_per_slot_step[2] = {0, 0};
+    bool tuned = false;
     double last_ms = 0;
     uint32_t last_launches = 0;
     ~State() {
@@ -830,6 +834,96 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
     return MM_OK;
 }
 
+// ---- Plans and the autotuner ------------------------------------------------------------------
+// The launch shape of a plan: the register kernel (n/8 threads; an exchange tile in global mode,
+// the whole vector and T resident otherwise) or the radix-4 kernel (one radix-4 group per
+// thread; the vector resident in LDS, or only the carry scratch in global mode).
+void set_plan(State* S, bool v2, bool global) {
+    const uint32_t n = S->n, k = S->k;
+    S->v2 = v2;
+    S->global = global;
+    if (v2) {
+        S->threads = n / 8;
+        const uint64_t tile = std::min<uint32_t>(n, n > 4096 ? 4096 : 2048);  // v2::Tiling
+        S->lds = global ? 24 * tile + lds_scratch_bytes(S->threads) : v2::v2_lds_resident(n, k, S->threads);
+    } else {
+        S->threads = plan_threads(n);
+        S->lds = global ? lds_scratch_bytes(S->threads) : lds_bytes_resident(n, k, S->threads);
+    }
+}
+
+// Whether a plan can run the batch: the register kernel needs the wraparound path and
+// n = 512 .. 8,192; an LDS-resident plan needs its bytes within the kernel's dynamic LDS.
+bool plan_fits(const State* S, bool v2, bool global) {
+    const Ctx* C = S->C;
+    if (v2 && !(S->wrap && S->n >= 512 && S->n <= 8192)) return false;
+    if (global) return true;
+    if (v2) return v2::v2_lds_resident(S->n, S->k, S->n / 8) <= C->max_dyn_lds_v2;
+    return lds_bytes_resident(S->n, S->k, plan_threads(S->n)) <= C->max_dyn_lds;
+}
+
+// Times every plan that fits for 16 steps on the batch's own slots and keeps the fastest. The
+// slots hold x = 0 and a 16-bit zero exponent for the timing (every kernel's work is the same
+// for any data, and 0 stays 0), and are cleared afterwards. The choice is cached per (n,
+// wraparound, slot-count octave), so later batches of that shape take it without timing.
+int autotune(State* S, bool v2ok) {
+    Ctx* C = S->C;
+    const auto key = std::make_tuple(S->n, S->wrap, ilog2(S->slots));
+    bool have = false, best_v2 = S->v2, best_global = S->global;
+    {
+        std::lock_guard<std::mutex> g(C->mu);
+        auto it = C->tuned.find(key);
+        if (it != C->tuned.end()) {
+            have = true;
+            best_v2 = it->second.first;
+            best_global = it->second.second;
+        }
+    }
+    if (!have) {
+        const uint32_t T = 16;
+        hipStream_t sm = C->stream;
+        std::vector<uint32_t> nb(S->slots, T);
+        HIPCHK(hipMemcpyAsync(S->d_nbits, nb.data(), S->slots * 4ull, hipMemcpyHostToDevice, sm));
+        double best = 1e300;
+        for (int v = v2ok ? 1 : 0; v >= 0; v--)
+            for (int g = 1; g >= 0; g--) {
+                if (!plan_fits(S, v, g)) continue;
+                set_plan(S, v, g);
+                double t = 1e300;
+                for (int rep = 0; rep < 2; rep++) {  // the first launch also loads the code
+                    HIPCHK(hipMemsetAsync(S->d_pos, 0, S->slots * 4ull, sm));
+                    HIPCHK(hipEventRecord(C->ev0, sm));
+                    if (v)
+                        hipLaunchKernelGGL(pick_step_v2(ilog2(S->n), g, false), dim3(S->slots), dim3(S->threads), S->lds, sm,
+                                           S->vargs(), 0u, T);
+                    else
+                        hipLaunchKernelGGL(pick_step(g, S->wrap, false), dim3(S->slots), dim3(S->threads), S->lds, sm,
+                                           S->args(), 0u, T);
+                    HIPCHK(hipGetLastError());
+                    HIPCHK(hipEventRecord(C->ev1, sm));
+                    HIPCHK(hipEventSynchronize(C->ev1));
+                    float ms = 0;
+                    hipEventElapsedTime(&ms, C->ev0, C->ev1);
+                    t = std::min(t, (double)ms);
+                }
+                if (t < best) {
+                    best = t;
+                    best_v2 = v;
+                    best_global = g;
+                }
+            }
+        HIPCHK(hipMemsetAsync(S->d_nbits, 0, S->slots * 4ull, sm));
+        HIPCHK(hipMemsetAsync(S->d_pos, 0, S->slots * 4ull, sm));
+        HIPCHK(hipMemsetAsync(S->d_err, 0, S->slots * 4ull, sm));
+        HIPCHK(hipStreamSynchronize(sm));
+        std::lock_guard<std::mutex> g(C->mu);
+        C->tuned[key] = {best_v2, best_global};
+    }
+    set_plan(S, best_v2, best_global);
+    S->tuned = true;
+    return 0;
+}
+
 // ---- Several lengths in one launch ----------------------------------------------------------
 // The length class of a batch in the mixed launch of nt threads: the register-transform step in
 // global mode (kind log2 n), or the radix-4 step, LDS-resident when its vector fits in 64 KiB.
@@ -990,7 +1084,7 @@ int mm_device_count(void) {
 int mm_hip_open(const mm_options* opts, mm_backend* out) {
     memset(out, 0, sizeof(*out));
     auto C = new Ctx();
-    C->opts = opts ? *opts : mm_options{0, 0, 1, 0};
+    C->opts = opts ? *opts : mm_options{0, 0, 1, 0, 0, 0};
     C->device = C->opts.device;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || C->device < 0 || C->device >= count) {
@@ -1061,27 +1155,21 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
     S->C = C;
     S->k = k; S->len = st->len; S->n = n; S->base = st->base; S->slots = st->slots; S->wrap = wrap;
     const char* kenv = getenv("MONTMUL_KERNEL");
-    S->v2 = wrap && n >= 512 && n <= 8192 && !(kenv && strcmp(kenv, "v1") == 0);
-    if (S->v2) {
-        S->threads = n / 8;
-        const uint64_t resident = v2::v2_lds_resident(n, k, S->threads);
+    const bool v2ok = plan_fits(S.get(), true, true) && !(kenv && strcmp(kenv, "v1") == 0);
+    if (v2ok) {
         // Global mode unless LDS is preferred: its smaller LDS footprint (T in HBM, exchange
         // tiles of at most 2,048 elements) fits more workgroups per CU, and that wins at every
         // length of this kernel (squarings x bits per second, 4,096 slots: L = 1,024: 3.73e11
         // against 3.21e11 resident; 2,048: 3.41 against 2.96; 4,096: 3.74 against 3.22;
         // 8,192: 3.3 M against 2.3 M squarings/s).
-        S->global = C->opts.force_global || resident > C->max_dyn_lds_v2 || !C->opts.prefer_lds;
-        const uint64_t tile = std::min<uint32_t>(n, n > 4096 ? 4096 : 2048);  // v2::Tiling
-        S->lds = S->global ? 24 * tile + lds_scratch_bytes(S->threads) : resident;
+        set_plan(S.get(), true, C->opts.force_global || !plan_fits(S.get(), true, false) || !C->opts.prefer_lds);
     } else {
-        S->threads = plan_threads(n);
-        const uint64_t resident = lds_bytes_resident(n, k, S->threads);
-        S->global = C->opts.force_global || resident > C->max_dyn_lds;
-        S->lds = S->global ? lds_scratch_bytes(S->threads) : resident;
+        set_plan(S.get(), false, C->opts.force_global || !plan_fits(S.get(), false, false));
     }
+    const bool tune = C->opts.autotune && !C->opts.force_global;
     // Device memory: x, N'-hat, N-hat or A|B, work vectors and limbs in global mode, exponents
     // of up to k words, per-slot words, tables (about 6n elements per field, twice), a margin.
-    const uint64_t per_slot = 4ull * k + 48ull * n + (S->global ? 24ull * n + 12ull * k : 0) + 16 + 4ull * k;
+    const uint64_t per_slot = 4ull * k + 48ull * n + (S->global || tune ? 24ull * n + 12ull * k : 0) + 16 + 4ull * k;
     const uint64_t need = (uint64_t)st->slots * per_slot + 300ull * n + 48ull * n * 2 + (64ull << 20);
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) return fail("hipMemGetInfo failed");
@@ -1094,8 +1182,9 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         dalloc(&S->d_nhat2, sl * n) || dalloc(&S->d_exp, sl))
         return 1;
     S->ew = 1;
-    if (S->global && (dalloc(&S->d_work1, sl * n) || dalloc(&S->d_work2, sl * n) || dalloc(&S->d_lim, sl * 3 * k))) return 1;
-    if (S->v2) {
+    if ((S->global || tune) && (dalloc(&S->d_work1, sl * n) || dalloc(&S->d_work2, sl * n) || dalloc(&S->d_lim, sl * 3 * k)))
+        return 1;
+    if (v2ok) {
         if (dalloc(&S->d_iwc1, n) || dalloc(&S->d_iwc2, n)) return 1;
         hipLaunchKernelGGL(k_scale_tab, dim3((n + 255) / 256), dim3(256), 0, C->stream, S->tab->vt.iw1, S->tab->vt.iw2,
                            st->base, n, S->d_iwc1, S->d_iwc2);
@@ -1140,6 +1229,7 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         }
         HIPCHK(hipStreamSynchronize(sm));
     }
+    if (tune && autotune(S.get(), v2ok)) return 1;
     S->h_x.assign(sl * k, 0);
     S->x_dirty.assign(sl, 0);
     S->exps.assign(sl, {});
@@ -1297,6 +1387,8 @@ int mm_plan(void* st, mm_plan_info* out) {
     out->threads = S->threads;
     out->lds_bytes = (uint32_t)S->lds;
     out->slots_per_wg = 1;
+    out->kernel = S->v2 ? 8 : 4;
+    out->tuned = S->tuned ? 1 : 0;
     return 0;
 }
 

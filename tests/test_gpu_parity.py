@@ -215,6 +215,29 @@ def test_graph_replayed_chunks_match_eager():
         eager.close()
 
 
+def test_autotuned_plans_match_default():
+    """The autotuner times every plan that fits (radix-8 register kernel or radix-4 kernel,
+    global or LDS-resident) on the batch's own slots and keeps the fastest: the chosen plan is
+    reported as tuned, a second batch of the same shape reuses it, and the results are the
+    default planner's, bit for bit."""
+    tuned = Hip(autotune=True)
+    try:
+        for bits, slots in ((3000, 64), (20_000, 64), (100_000, 32)):
+            rng = random.Random(bits)
+            moduli = [common.rand_odd(rng, bits) for _ in range(slots)]
+            b1, b2 = ModBatch(tuned, moduli, 3), ModBatch(tuned, moduli, 3)
+            p1, p2 = Hip.plan_of(b1), Hip.plan_of(b2)
+            assert p1["tuned"] and p1 == p2, (bits, p1, p2)
+            assert p1["kernel"] in (4, 8)
+            if b1.params().len < 1024:
+                assert p1["kernel"] == 4
+            b1.free()
+            b2.free()
+            common.assert_same_raw(lambda: tuned, lambda: Hip(), bits, 16, 96, seed=bits)
+    finally:
+        tuned.close()
+
+
 def test_two_batches_interleaved(hip):
     """Two live batches with different L, run alternately (tests/gpu.rs:227-258)."""
     rng = random.Random(13)
