@@ -162,6 +162,28 @@ def gmp_step(x, n, nprime, k, c):
     return int.from_bytes(out.tobytes(), "little")
 
 
+def _limbs(v, k=None):
+    k = k if k is not None else max(1, (v.bit_length() + 31) // 32)
+    return np.frombuffer(v.to_bytes(4 * k, "little"), dtype=np.uint32).copy()
+
+
+def gmp_powm(b, e, m):
+    """b^e mod m by GMP's mpz_powm (b, e >= 0)."""
+    bs, es, ms = _limbs(b), _limbs(e), _limbs(m)
+    out = np.zeros(len(ms), dtype=np.uint32)
+    olib.og_powm(_p32(bs), len(bs), _p32(es), len(es), _p32(ms), len(ms), _p32(out))
+    return int.from_bytes(out.tobytes(), "little")
+
+
+def gmp_invert(a, m):
+    """a^-1 mod m by GMP's mpz_invert; ValueError when there is none."""
+    as_, ms = _limbs(a), _limbs(m)
+    out = np.zeros(len(ms), dtype=np.uint32)
+    if not olib.og_invert(_p32(as_), len(as_), _p32(ms), len(ms), _p32(out)):
+        raise ValueError("not invertible")
+    return int.from_bytes(out.tobytes(), "little")
+
+
 class CpuTransform:
     """The oracle's transform layer (Cpu's Transform impl, cpu.rs:349-407) behind the adapter
     interface of common.exactness / common.transform_batched."""
