@@ -847,7 +847,7 @@ void og_step(const uint32_t* x, const uint32_t* n, const uint32_t* nprime, size_
     mpz_to_limbs(s.x, out, k);
 }
 
-// GMP's powm and invert on little-endian u32 limbs: the tests' expected values at sizes where
+// GMP's powm, product mod m and invert on little-endian u32 limbs: the tests' expected values at sizes where
 // Python integers are slow (the reference's tests take them from rug's pow_mod and invert).
 // out holds nm limbs. og_invert returns 0 when a has no inverse mod m.
 void og_powm(const uint32_t* b, size_t nb, const uint32_t* e, size_t ne, const uint32_t* m, size_t nm, uint32_t* out) {
@@ -857,6 +857,16 @@ void og_powm(const uint32_t* b, size_t nb, const uint32_t* e, size_t ne, const u
     mpz_powm(R, B, E, M);
     mpz_to_limbs(R, out, nm);
     mpz_clears(B, E, M, R, nullptr);
+}
+
+void og_mulmod(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, const uint32_t* m, size_t nm, uint32_t* out) {
+    mpz_t A, B, M;
+    mpz_inits(A, B, M, nullptr);
+    limbs_to_mpz(A, a, na); limbs_to_mpz(B, b, nb); limbs_to_mpz(M, m, nm);
+    mpz_mul(A, A, B);
+    mpz_mod(A, A, M);
+    mpz_to_limbs(A, out, nm);
+    mpz_clears(A, B, M, nullptr);
 }
 
 int og_invert(const uint32_t* a, size_t na, const uint32_t* m, size_t nm, uint32_t* out) {

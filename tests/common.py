@@ -24,6 +24,15 @@ def powmod(b, e, m):
     return gmp_powm(b % m, e, m)
 
 
+def mulmod(a, b, m):
+    """a b mod m, through GMP above 2,048 bits (Python's division is quadratic there)."""
+    if m.bit_length() <= 2048:
+        return a * b % m
+    from oracle_backends import gmp_mulmod
+
+    return gmp_mulmod(a % m, b % m, m)
+
+
 def rand_odd(rng, bits):
     """Random odd integer of exactly `bits` bits (common/mod.rs:10-15)."""
     return rng.getrandbits(bits) | (1 << (bits - 1)) | 1
@@ -32,7 +41,7 @@ def rand_odd(rng, bits):
 def expected(x, c, n, k):
     """x^2 c R^-1 mod N, the expected step result (common/mod.rs:17-21)."""
     rinv = powmod(1 << (32 * k), -1, n)
-    return x * x * c * rinv % n
+    return mulmod(powmod(x, 2, n), c * rinv, n)
 
 
 def tribonacci(n):
@@ -242,7 +251,7 @@ def api_contract(make, seed=6):
             b.run(1 << 20)
             assert b.remaining(0) == 0
             sq = 1 << e.bit_length()
-            want = powmod(x % n, sq, n) * powmod(3, e, n) % n
+            want = mulmod(powmod(x % n, sq, n), powmod(3, e, n), n)
             assert b.read(0) == want, f"pow, n {n}"
     moduli = [rand_odd(rng, 777) for _ in range(5)]
     exps = [rng.getrandbits(10 + 60 * i) | 1 for i in range(5)]
@@ -286,7 +295,7 @@ def carry_stress(make, max_len):
         for i, m in enumerate((n, n2)):
             rinv = powmod(1 << (32 * k), -1, m)
             x = (m - rinv) % m
-            want = powmod(x, 1 << 32, m) * powmod(3, e, m) % m
+            want = mulmod(powmod(x, 1 << 32, m), powmod(3, e, m), m)
             assert b.read(i) == want, f"len {length} slot {i}"
         length *= 2
 
@@ -452,7 +461,7 @@ def mixed_lengths(make, bit_sizes, ebits, chunk, seed=9):
         assert rounds <= ebits // chunk + 2
     for i, n in enumerate(moduli):
         nb = exps[i].bit_length()
-        assert mix.read(i) == powmod(xs[i], 1 << nb, n) * powmod(3, exps[i], n) % n, f"slot {i}, {n.bit_length()} bits"
+        assert mix.read(i) == mulmod(powmod(xs[i], 1 << nb, n), powmod(3, exps[i], n), n), f"slot {i}, {n.bit_length()} bits"
     for length in lengths:
         idx = [i for i in range(len(moduli)) if mix.params(i).len == length]
         plain = ModBatch(make(), [moduli[i] for i in idx], 3)

@@ -175,6 +175,14 @@ def gmp_powm(b, e, m):
     return int.from_bytes(out.tobytes(), "little")
 
 
+def gmp_mulmod(a, b, m):
+    """a b mod m by GMP (a, b >= 0)."""
+    as_, bs, ms = _limbs(a), _limbs(b), _limbs(m)
+    out = np.zeros(len(ms), dtype=np.uint32)
+    olib.og_mulmod(_p32(as_), len(as_), _p32(bs), len(bs), _p32(ms), len(ms), _p32(out))
+    return int.from_bytes(out.tobytes(), "little")
+
+
 def gmp_invert(a, m):
     """a^-1 mod m by GMP's mpz_invert; ValueError when there is none."""
     as_, ms = _limbs(a), _limbs(m)
