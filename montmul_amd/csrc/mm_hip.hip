@@ -929,15 +929,18 @@ int autotune(State* S, bool v2ok) {
 // global mode (kind log2 n), or the radix-4 step, LDS-resident when its vector fits in 64 KiB.
 // Allocates the HBM buffers that mode needs and the batch's own plan did not. Writes the LDS
 // bytes a workgroup needs.
-int mix_class(State* S, uint32_t nt, MixClass* out, uint64_t* lds) {
+int mix_class(State* S, uint32_t nt, MixClass* out, uint64_t* lds, uint32_t* groups) {
     const uint32_t n = S->n, k = S->k;
     const size_t sl = S->slots;
     memset(out, 0, sizeof(*out));
+    *groups = 1;
     if (S->v2) {
         if (!S->d_lim && dalloc(&S->d_lim, sl * 3 * k)) return 1;
         out->kind = ilog2(n);
         out->v = S->vargs();
-        *lds = 24ull * std::min<uint32_t>(n, n > 4096 ? 4096 : 2048) + lds_scratch_bytes(nt);  // v2::Tiling
+        *groups = mix_groups(nt, out->kind);
+        *lds = *groups > 1 ? (uint64_t)*groups * mix_group_lds(out->kind)
+                           : 24ull * std::min<uint32_t>(n, n > 4096 ? 4096 : 2048) + lds_scratch_bytes(nt);  // v2::Tiling
         return 0;
     }
     const uint64_t resident = lds_bytes_resident(n, k, nt);
@@ -972,10 +975,11 @@ int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which)
     uint64_t left = std::min(bits, most);
     if (left == 0) return MM_OK;
     std::vector<MixClass> cls(count);
+    std::vector<uint32_t> groups(count, 1);
     uint64_t lds = 0;
     for (size_t g = 0; g < count; g++) {
         uint64_t l = 0;
-        if (push(G[g]) || mix_class(G[g], nt, &cls[g], &l)) return MM_EFAULT;
+        if (push(G[g]) || mix_class(G[g], nt, &cls[g], &l, &groups[g])) return MM_EFAULT;
         lds = std::max(lds, l);
     }
     const size_t lim = C->max_dyn_lds_mix[nt == 1024];
@@ -988,24 +992,33 @@ int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which)
     const uint32_t per_cu = nt == 1024 ? 1 : (2 * lds <= C->max_lds ? 2 : 1);
     hipStream_t sm = C->stream;
     std::vector<MixItem> items;
+    std::vector<uint32_t> list;
     std::vector<double> cost;
     while (left > 0) {
         uint64_t s = left;
         if (C->opts.max_launch_ms > 0)
             s = C->mix_ms_per_step > 0 ? (uint64_t)std::max(1.0, C->opts.max_launch_ms / C->mix_ms_per_step) : 16;
         s = std::min<uint64_t>(std::min(s, left), 0xffffffffu);
-        // Work items: every slot with bits left, the most work first (longest-processing-time
+        // Work items: the slots with bits left, in runs of up to `groups` slots of one length
+        // (slots with the most steps together), the most work first (longest-processing-time
         // order: the longest chains start at once, short ones fill in behind them).
         items.clear();
+        list.clear();
         cost.clear();
         for (size_t g = 0; g < count; g++) {
             const State* S = G[g];
             const double per_step = (double)S->n * ilog2(S->n) * (S->v2 ? 1.0 : 3.0);
+            std::vector<std::pair<uint64_t, uint32_t>> todo;  // (steps, slot)
             for (uint32_t i = 0; i < S->slots; i++) {
                 const uint64_t r = std::min<uint64_t>(s, S->nbits[i] - S->pos[i]);
-                if (r == 0) continue;
-                items.push_back(MixItem{(uint32_t)g, i});
-                cost.push_back((double)r * per_step);
+                if (r > 0) todo.push_back({r, i});
+            }
+            std::stable_sort(todo.begin(), todo.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+            for (size_t a = 0; a < todo.size(); a += groups[g]) {
+                const size_t b = std::min(todo.size(), a + groups[g]);
+                items.push_back(MixItem{(uint32_t)g, (uint32_t)list.size(), (uint32_t)(b - a), (uint32_t)todo[a].first});
+                for (size_t j = a; j < b; j++) list.push_back(todo[j].second);
+                cost.push_back((double)todo[a].first * per_step);
             }
         }
         std::vector<uint32_t> order(items.size());
@@ -1014,7 +1027,8 @@ int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which)
         std::vector<MixItem> queue(items.size());
         for (size_t i = 0; i < order.size(); i++) queue[i] = items[order[i]];
         const size_t cls_off = 256, items_off = cls_off + ((count * sizeof(MixClass) + 255) & ~(size_t)255);
-        const size_t need = items_off + queue.size() * sizeof(MixItem);
+        const size_t list_off = items_off + ((queue.size() * sizeof(MixItem) + 255) & ~(size_t)255);
+        const size_t need = list_off + list.size() * sizeof(uint32_t);
         if (need > C->mix_cap) {
             if (hipStreamSynchronize(sm) != hipSuccess) { fail("stream synchronize failed"); return MM_EFAULT; }
             if (C->mix_buf) hipFree(C->mix_buf);
@@ -1026,9 +1040,12 @@ int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which)
         uint32_t* head = (uint32_t*)C->mix_buf;
         const MixClass* dcls = (const MixClass*)(C->mix_buf + cls_off);
         const MixItem* ditems = (const MixItem*)(C->mix_buf + items_off);
+        const uint32_t* dlist = (const uint32_t*)(C->mix_buf + list_off);
         hipError_t ce = hipMemcpyAsync(C->mix_buf + cls_off, cls.data(), count * sizeof(MixClass), hipMemcpyHostToDevice, sm);
         if (ce == hipSuccess)
             ce = hipMemcpyAsync(C->mix_buf + items_off, queue.data(), queue.size() * sizeof(MixItem), hipMemcpyHostToDevice, sm);
+        if (ce == hipSuccess)
+            ce = hipMemcpyAsync(C->mix_buf + list_off, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice, sm);
         if (ce != hipSuccess) { hip_fail("mixed launch upload", ce); return MM_EFAULT; }
         // The upload must land before a replayed graph reads it.
         if (hipStreamSynchronize(sm) != hipSuccess) { fail("mixed launch upload failed"); return MM_EFAULT; }
@@ -1042,8 +1059,8 @@ int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which)
                            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sm, head, 1u, 0u);
                            hipError_t fe = hipGetLastError();
                            if (fe != hipSuccess) return fe;
-                           return nt == 1024 ? mix_launch<1024>(grid, lds, sm, dcls, ditems, (uint32_t)queue.size(), head, (uint32_t)s)
-                                             : mix_launch<512>(grid, lds, sm, dcls, ditems, (uint32_t)queue.size(), head, (uint32_t)s);
+                           return nt == 1024 ? mix_launch<1024>(grid, lds, sm, dcls, ditems, dlist, (uint32_t)queue.size(), head)
+                                             : mix_launch<512>(grid, lds, sm, dcls, ditems, dlist, (uint32_t)queue.size(), head);
                        },
                        &ms))
             return MM_EFAULT;

@@ -11,13 +11,20 @@
 namespace mm {
 
 template <uint32_t NT, uint32_t LOGN>
-__device__ __forceinline__ void mix_v2(const v2::VArgs& A, uint32_t slot, uint32_t bits, unsigned char* lds) {
-    if constexpr ((1u << LOGN) / 8 <= NT) steps_v2<LOGN, true, false, ((1u << LOGN) / 8 < NT)>(A, slot, bits, lds);
+__device__ __forceinline__ void mix_v2(const v2::VArgs& A, const uint32_t* list, const MixItem& it, unsigned char* lds) {
+    constexpr uint32_t G = mix_groups(NT, LOGN);
+    if constexpr ((1u << LOGN) / 8 > NT)
+        return;
+    else if constexpr (G > 1)
+        steps_v2_group<LOGN, G>(A, list, it.first, it.count, it.steps, lds);
+    else
+        steps_v2<LOGN, true, false, ((1u << LOGN) / 8 < NT)>(A, list[it.first], it.steps, lds);
 }
 
 template <uint32_t NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_PER_EU))) void k_mixed(
-    const MixClass* __restrict__ cls, const MixItem* __restrict__ items, uint32_t count, uint32_t* head, uint32_t bits) {
+    const MixClass* __restrict__ cls, const MixItem* __restrict__ items, const uint32_t* __restrict__ list, uint32_t count,
+    uint32_t* head) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t next;
     for (;;) {
@@ -34,16 +41,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_
         if (i >= count) break;
         const MixItem it = items[i];
         const MixClass& M = cls[it.cls];
+        const uint32_t slot = list[it.first];
         switch (M.kind) {
-            case 9: mix_v2<NT, 9>(M.v, it.slot, bits, lds); break;
-            case 10: mix_v2<NT, 10>(M.v, it.slot, bits, lds); break;
-            case 11: mix_v2<NT, 11>(M.v, it.slot, bits, lds); break;
-            case 12: mix_v2<NT, 12>(M.v, it.slot, bits, lds); break;
-            case 13: mix_v2<NT, 13>(M.v, it.slot, bits, lds); break;
-            case MIX_V1: steps_v1<false, false, false>(M.k, it.slot, bits, lds); break;
-            case MIX_V1 | MIX_V1_WRAP: steps_v1<false, true, false>(M.k, it.slot, bits, lds); break;
-            case MIX_V1 | MIX_V1_GLOBAL: steps_v1<true, false, false>(M.k, it.slot, bits, lds); break;
-            case MIX_V1 | MIX_V1_GLOBAL | MIX_V1_WRAP: steps_v1<true, true, false>(M.k, it.slot, bits, lds); break;
+            case 9: mix_v2<NT, 9>(M.v, list, it, lds); break;
+            case 10: mix_v2<NT, 10>(M.v, list, it, lds); break;
+            case 11: mix_v2<NT, 11>(M.v, list, it, lds); break;
+            case 12: mix_v2<NT, 12>(M.v, list, it, lds); break;
+            case 13: mix_v2<NT, 13>(M.v, list, it, lds); break;
+            case MIX_V1: steps_v1<false, false, false>(M.k, slot, it.steps, lds); break;
+            case MIX_V1 | MIX_V1_WRAP: steps_v1<false, true, false>(M.k, slot, it.steps, lds); break;
+            case MIX_V1 | MIX_V1_GLOBAL: steps_v1<true, false, false>(M.k, slot, it.steps, lds); break;
+            case MIX_V1 | MIX_V1_GLOBAL | MIX_V1_WRAP: steps_v1<true, true, false>(M.k, slot, it.steps, lds); break;
             default: break;
         }
     }
@@ -51,8 +59,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_
 
 template <>
 hipError_t mix_launch<MM_MIX_NT>(uint32_t grid, size_t lds, hipStream_t sm, const MixClass* cls, const MixItem* items,
-                                 uint32_t count, uint32_t* head, uint32_t bits) {
-    hipLaunchKernelGGL(k_mixed<MM_MIX_NT>, dim3(grid), dim3(MM_MIX_NT), lds, sm, cls, items, count, head, bits);
+                                 const uint32_t* list, uint32_t count, uint32_t* head) {
+    hipLaunchKernelGGL(k_mixed<MM_MIX_NT>, dim3(grid), dim3(MM_MIX_NT), lds, sm, cls, items, list, count, head);
     return hipGetLastError();
 }
 

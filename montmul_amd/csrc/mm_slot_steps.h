@@ -107,10 +107,12 @@ __device__ __forceinline__ void steps_v2(const v2::VArgs& A, uint32_t slot, uint
 
 // ---- Slots of several lengths in one persistent launch -------------------------------------
 // A group of batches, one per length, is stepped by one launch of NT-thread workgroups. The
-// work queue is an array of per-slot descriptors (length class, slot) in device memory, sorted
-// longest first; every workgroup takes the next descriptor with an atomic add on a head counter
-// and runs that slot's steps, until the queue is empty. No reference counterpart: the
-// reference's batches hold one length each (batch.rs:64-77).
+// work queue is an array of descriptors in device memory, sorted longest first: a length class,
+// a run of `count` slots in a slot list, and a step count. Every workgroup takes the next
+// descriptor with an atomic add on a head counter and steps its slots, until the queue is
+// empty. A register-transform length whose n / 8 threads fill a fraction of the workgroup runs
+// several slots side by side, one per thread group. No reference counterpart: the reference's
+// batches hold one length each (batch.rs:64-77).
 constexpr uint32_t MIX_V1 = 0x100, MIX_V1_GLOBAL = 0x200, MIX_V1_WRAP = 0x400;
 struct MixClass {
     v2::VArgs v;    // register-transform lengths: kind = log2 n (9 .. 13)
@@ -118,15 +120,72 @@ struct MixClass {
     uint32_t kind;
 };
 struct MixItem {
-    uint32_t cls, slot;
+    uint32_t cls, first, count, steps;  // slots list[first .. first + count), up to `steps` steps each
 };
+
+// LDS of one slot group of the register-transform step at log2 n = LOGN in global mode: the
+// exchange tile (24 bytes per element) and the carry scratch of its n / 8 threads, 16-aligned.
+__host__ __device__ constexpr uint32_t mix_group_lds(uint32_t logn) {
+    return ((24u * ((1u << logn) > (logn > 12 ? 4096u : 2048u) ? (logn > 12 ? 4096u : 2048u) : (1u << logn)) +
+             8u * (2u * (((1u << logn) / 8) / 64) + 2u)) + 15u) & ~15u;
+}
+// Slots per workgroup of NT threads at log2 n = LOGN: a power of two, at most NT / (n / 8), and
+// within the LDS of one workgroup per CU (NT = 1,024) or of two (NT = 512) on a 160 KiB CU.
+__host__ __device__ constexpr uint32_t mix_groups(uint32_t nt, uint32_t logn) {
+    const uint32_t t = (1u << logn) / 8;
+    if (t >= nt) return 1;
+    const uint32_t cap = nt >= 1024 ? 159u * 1024u : 79u * 1024u;
+    uint32_t g = nt / t;
+    while (g > 1 && g * mix_group_lds(logn) > cap) g /= 2;
+    return g;
+}
+
+// G slots of one register-transform length in one workgroup of G n/8 threads: thread group g
+// steps slot list[first + g] (g < count; the other groups join every barrier and touch no
+// memory) for up to `steps` steps, each group with its own exchange tile and carry scratch.
+template <uint32_t LOGN, uint32_t G>
+__device__ __forceinline__ void steps_v2_group(const v2::VArgs& A, const uint32_t* list, uint32_t first, uint32_t count,
+                                               uint32_t steps, unsigned char* lds) {
+    constexpr uint32_t T = (1u << LOGN) / 8, tile = v2::Tiling<LOGN, true>::tile;
+    const Slots& S = A.s;
+    const uint32_t n = 1u << LOGN, k = S.k;
+    const uint32_t g = __builtin_amdgcn_readfirstlane(threadIdx.x / T);  // whole waves per group
+    const bool on = g < count;
+    const uint32_t slot = list[first + (on ? g : 0)];
+    unsigned char* base = lds + g * mix_group_lds(LOGN);
+    v2::Xbuf X;
+    X.z2 = (v2::Z2*)base;
+    X.z1 = (v2::Z1*)(base + 16 * tile);
+    uint64_t* sc = (uint64_t*)(base + 24 * tile);
+    uint32_t* Tl = S.lim + (size_t)slot * 3 * k;
+    uint32_t* x = S.x + (size_t)slot * k;
+    const uint32_t* ex = S.exp + (size_t)slot * S.ew;
+    const uint32_t nb = S.nbits[slot];
+    uint32_t pos = S.pos[slot];
+    uint32_t bad = 0;
+    for (uint32_t it = 0; it < steps; it++) {
+        __syncthreads();  // x written by every thread in the previous step
+        const bool act = on && pos < nb;
+        const uint32_t bit = act ? (ex[pos >> 5] >> (31 - (pos & 31))) & 1 : 0;
+        const F1* iw1 = bit ? A.iwc1 : A.t.iw1;
+        const v2::S2* iw2 = bit ? A.iwc2 : A.t.iw2;
+        bad |= v2::v2_square<LOGN, true, true, T>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv, act);
+        bad |= v2::v2_redc<LOGN, true, true, T>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
+                                                  S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n, act);
+        if (act) pos++;
+    }
+    if (on && (threadIdx.x & (T - 1)) == 0) {
+        S.pos[slot] = pos;
+        if (bad) S.err[slot] = 1;
+    }
+}
 
 // The mixed-length kernel of NT threads per workgroup (512 or 1,024; one object file each):
 // launches it on `sm`, and sets its dynamic LDS limit to what the device allows beside its
 // static LDS (written to *dyn).
 template <uint32_t NT>
-hipError_t mix_launch(uint32_t grid, size_t lds, hipStream_t sm, const MixClass* cls, const MixItem* items, uint32_t count,
-                      uint32_t* head, uint32_t bits);
+hipError_t mix_launch(uint32_t grid, size_t lds, hipStream_t sm, const MixClass* cls, const MixItem* items,
+                      const uint32_t* list, uint32_t count, uint32_t* head);
 template <uint32_t NT> hipError_t mix_lds_limit(size_t max_lds, size_t* dyn);
 
 }  // namespace mm

@@ -346,6 +346,13 @@ __device__ __forceinline__ u32 opaque_tid() {
     return t;
 }
 
+// The thread's index within its slot's thread group: the whole workgroup (GT = 0), or groups of
+// GT threads (a power of two, whole waves) when several slots share a workgroup.
+template <u32 GT> __device__ __forceinline__ u32 slot_tid() {
+    const u32 t = opaque_tid();
+    return GT ? (t & (GT - 1)) : t;
+}
+
 // Pins a value in registers at this point: the compiler may not sink its computation past
 // later barriers (where it would keep the larger inputs live instead).
 __device__ __forceinline__ void pin(u32& v) { asm volatile("" : "+v"(v)); }
@@ -439,15 +446,16 @@ __device__ __forceinline__ void exchange(u32 t, bool act, Z1* v1, Z2* v2, const 
 }
 
 // Full DIF on registers laid out for pass 0; ends laid out for the last pass (positions
-// 8t .. 8t+7). Threads t >= n / 8 hold no elements but join every barrier.
-template <u32 LOGN, bool TILED>
-__device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, const S2* tw2, const Xbuf& X) {
+// 8t .. 8t+7). Threads t >= n / 8 hold no elements but join every barrier, and so do all
+// threads when `on` is false (an idle slot group). GT: see slot_tid.
+template <u32 LOGN, bool TILED, u32 GT = 0>
+__device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, const S2* tw2, const Xbuf& X, bool on = true) {
     typedef CPlan<LOGN> P;
-    const bool act = t < P::T;
+    const bool act = on && t < P::T;
 #pragma unroll
     for (u32 p = 0; p < P::npass; p++) {
         const u32 r = P::radix(p), ls = P::lstride(p);
-        t = opaque_tid();
+        t = slot_tid<GT>();
         if (act) {
             bfly<Op1, true>(v1, r);
             twiddle<Op1>(v1, tw1 + P::tw_off(p), t, r, ls, P::T);
@@ -462,15 +470,15 @@ __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, c
 }
 
 // Full DIT from the last pass's layout back to pass 0's (natural positions).
-template <u32 LOGN, bool TILED>
-__device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, const S2* itw2, const Xbuf& X) {
+template <u32 LOGN, bool TILED, u32 GT = 0>
+__device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, const S2* itw2, const Xbuf& X, bool on = true) {
     typedef CPlan<LOGN> P;
-    const bool act = t < P::T;
+    const bool act = on && t < P::T;
 #pragma unroll
     for (u32 q = 0; q < P::npass; q++) {
         const u32 p = P::npass - 1 - q;
         const u32 r = P::radix(p), ls = P::lstride(p);
-        t = opaque_tid();
+        t = slot_tid<GT>();
         if (act) {
             twiddle<Op1>(v1, itw1 + P::tw_off(p), t, r, ls, P::T);
             bfly<Op1, false>(v1, r);

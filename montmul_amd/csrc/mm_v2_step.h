@@ -32,15 +32,17 @@ __host__ __device__ inline uint64_t v2_lds_resident(uint32_t n, uint32_t k, uint
 // WIDE: the workgroup may have more threads than the vector's n / 8; threads t >= n / 8 join
 // every barrier and touch no memory (the mixed-length launch runs every length in workgroups
 // of the largest length's size). Without WIDE, blockDim = n / 8 and no thread is idle.
-#define MM_ACT(t) (!WIDE || (t) < T)
+// GT > 0: the workgroup holds several slots, one per group of GT = n / 8 threads; t is the
+// index within the group, and a group whose `on` is false (no slot) touches no memory.
+#define MM_ACT(t) (on && (!WIDE || (t) < T))
 
 // One step's squaring: T = c x^2 into Tl (2k limbs); returns nonzero on a carry out.
 // iw1, iw2: the unweighting, zeta^-j / n or, on a 1 bit, zeta^-j c / n.
-template <u32 LOGN, bool TILED, bool WIDE = false>
+template <u32 LOGN, bool TILED, bool WIDE = false, u32 GT = 0>
 __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, u64* sc, const u32* x, const F1* iw1,
-                                         const S2* iw2, u32 inv) {
+                                         const S2* iw2, u32 inv, bool on = true) {
     typedef CPlan<LOGN> P;
-    u32 t = opaque_tid();  // refreshed per phase: no address held across phases
+    u32 t = slot_tid<GT>();  // refreshed per phase: no address held across phases
     const u32 T = P::T, n = P::n;
     constexpr u32 NT = WIDE ? P::T : 0;
     const u32 r0 = P::radix(0), ls0 = P::lstride(0);
@@ -56,14 +58,14 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
             MM_GROUP_FENCE(e);
         }
     }
-    dif_regs<LOGN, TILED>(t, v1, v2, V.tw1, V.tw2, X);
+    dif_regs<LOGN, TILED, GT>(t, v1, v2, V.tw1, V.tw2, X, on);
 #pragma unroll
     for (u32 e = 0; e < 8; e++) {
         v1[e] = Op1::sqr(v1[e]);
         v2[e] = Op2::sqr(v2[e]);
     }
-    dit_regs<LOGN, TILED>(t, v1, v2, V.itw1, V.itw2, X);
-    t = opaque_tid();
+    dit_regs<LOGN, TILED, GT>(t, v1, v2, V.itw1, V.itw2, X, on);
+    t = slot_tid<GT>();
     typedef Tiling<LOGN, TILED> G;
     u32 l[16];
     u64 top;
@@ -82,7 +84,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
             }
         }
         __syncthreads();
-        t = opaque_tid();
+        t = slot_tid<GT>();
         top = carry_regs<16, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 16 * t + i) : (u128)0; }, l, sc, 0);
     } else {
         // The tile holds 2 * tile = n coefficients: the real parts (coefficients 0 .. n-1) go
@@ -109,7 +111,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         }
         __syncthreads();
         u32 lo[8];
-        t = opaque_tid();
+        t = slot_tid<GT>();
         const u64 mid = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) : (u128)0; }, lo, sc, 0);
         if (MM_ACT(t)) {
 #pragma unroll
@@ -119,7 +121,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         }
         __syncthreads();
         u32 hi[8];
-        t = opaque_tid();
+        t = slot_tid<GT>();
         const u64 top = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) : (u128)0; }, hi, sc, mid);
         if (MM_ACT(t)) {
 #pragma unroll
@@ -137,14 +139,14 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
 // Reduction: x = T R^-1 (lazy) from T in Tl, through m = (T mod R) N' mod R and the cyclic
 // product m N mod (R - 1) (montmul/src/cpu.rs:452-484, 297-328). Returns nonzero on a carry
 // that must vanish but does not.
-template <u32 LOGN, bool TILED, bool WIDE = false>
+template <u32 LOGN, bool TILED, bool WIDE = false, u32 GT = 0>
 __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u64* sc, u32* x, const F1* np1, const F2* np2,
-                                       const F1* ab1, const F2* ab2) {
+                                       const F1* ab1, const F2* ab2, bool on = true) {
     const VTab& V = A.t;
     const VTab& H = A.ht;
     typedef CPlan<LOGN> P;
     typedef CPlan<LOGN - 1> PH;
-    u32 t = opaque_tid();  // refreshed per phase: no address held across phases
+    u32 t = slot_tid<GT>();  // refreshed per phase: no address held across phases
     const u32 T = P::T, n = P::n, k = n, h = n / 2, Th = PH::T;
     constexpr u32 NT = WIDE ? P::T : 0;
     const u32 r0 = P::radix(0), ls0 = P::lstride(0);
@@ -165,8 +167,8 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
             MM_GROUP_FENCE(e);
         }
     }
-    dif_regs<LOGN, TILED>(t, v1, v2, V.tw1, V.tw2, X);
-    t = opaque_tid();
+    dif_regs<LOGN, TILED, GT>(t, v1, v2, V.tw1, V.tw2, X, on);
+    t = slot_tid<GT>();
     if (MM_ACT(t)) {
 #pragma unroll
         for (u32 e = 0; e < 8; e++) {
@@ -175,9 +177,9 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
             MM_GROUP_FENCE(e);
         }
     }
-    dit_regs<LOGN, TILED>(t, v1, v2, V.itw1, V.itw2, X);
+    dit_regs<LOGN, TILED, GT>(t, v1, v2, V.itw1, V.itw2, X, on);
     __syncthreads();
-    t = opaque_tid();
+    t = slot_tid<GT>();
     if (MM_ACT(t)) {
 #pragma unroll
         for (u32 e = 0; e < 8; e++) {
@@ -189,10 +191,10 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         }
     }
     __syncthreads();
-    t = opaque_tid();
+    t = slot_tid<GT>();
     u32 m[8];
     carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) : (u128)0; }, m, sc, 0);
-    t = opaque_tid();
+    t = slot_tid<GT>();
     // m packed as z_j = m_2j + i m_2j+1 at half-length positions j = 4t .. 4t+3.
     if (MM_ACT(t)) {
 #pragma unroll
@@ -203,11 +205,11 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         }
     }
     __syncthreads();
-    t = opaque_tid();
-    const bool hact = t < Th;
+    t = slot_tid<GT>();
+    const bool hact = on && t < Th;
     if (hact) get(X, v1, v2, t, PH::radix(0), PH::lstride(0), Th);
-    dif_regs<LOGN - 1, TILED>(t, v1, v2, H.tw1, H.tw2, X);
-    t = opaque_tid();
+    dif_regs<LOGN - 1, TILED, GT>(t, v1, v2, H.tw1, H.tw2, X, on);
+    t = slot_tid<GT>();
     // Pointwise with the partner position: out_p = (z_p + conj z_q) A_p + (z_p - conj z_q) B_p.
     __syncthreads();
     if (hact) {
@@ -219,7 +221,7 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         }
     }
     __syncthreads();
-    t = opaque_tid();
+    t = slot_tid<GT>();
     if (hact) {
 #pragma unroll
         for (u32 e = 0; e < 8; e++) {
@@ -233,9 +235,9 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
             MM_GROUP_FENCE(e);
         }
     }
-    t = opaque_tid();
-    dit_regs<LOGN - 1, TILED>(t, v1, v2, H.itw1, H.itw2, X);
-    t = opaque_tid();
+    t = slot_tid<GT>();
+    dit_regs<LOGN - 1, TILED, GT>(t, v1, v2, H.itw1, H.itw2, X, on);
+    t = slot_tid<GT>();
     // Cyclic product element j carries coefficients 2j (re) and 2j + 1 (im).
     __syncthreads();
     if (hact) {
@@ -248,23 +250,26 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
         }
     }
     __syncthreads();
-    t = opaque_tid();
+    t = slot_tid<GT>();
     // G = (cyclic + T mod R) mod (R - 1), x = floor(T / R) + G.
     u32 xs[8];
     u64 cc = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) + Tl[8 * t + i] : (u128)0; }, xs, sc, 0);
+    // With several slot groups in the workgroup the fold count must be the same in all of them
+    // (the carry holds barriers): they fold while any group has a carry left; a further fold
+    // with carry 0 leaves a group's limbs as they are.
 #pragma nounroll
-    for (int guard = 0; cc != 0 && guard < 4; guard++) {
+    for (int guard = 0; guard < 4 && (GT ? __syncthreads_or(cc != 0) != 0 : cc != 0); guard++) {
         u32 prev[8];
 #pragma unroll
         for (u32 i = 0; i < 8; i++) prev[i] = xs[i];
         cc = carry_regs<8, NT>(t, [&](int i) { return (u128)prev[i]; }, xs, sc, cc);
     }
-    t = opaque_tid();
+    t = slot_tid<GT>();
     u32 lo[8];
 #pragma unroll
     for (u32 i = 0; i < 8; i++) lo[i] = xs[i];
     u64 top = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? (u128)lo[i] + Tl[k + 8 * t + i] : (u128)0; }, xs, sc, 0);
-    t = opaque_tid();
+    t = slot_tid<GT>();
     if (MM_ACT(t)) {
 #pragma unroll
         for (u32 i = 0; i < 8; i++) x[8 * t + i] = xs[i];

@@ -74,3 +74,12 @@ def test_mixed_fermat_planted(hip_global):
         assert b.is_one(i) == (common.powmod(3, n - 1, n) == 1), f"slot {i}"
     assert all(b.is_one(i) for i in range(len(ms)))
     b.free()
+
+
+def test_mixed_full_slot_groups(hip_global):
+    """Enough slots per length that the short register-transform lengths fill whole slot
+    groups (several slots per workgroup) and leave partial ones, with exponents of different
+    lengths, so groups go idle at different steps."""
+    sizes = [6000] * 17 + [12000] * 9 + [20000] * 10 + [40000] * 5 + [100000] * 5 + [150000]
+    lengths = common.mixed_lengths(lambda: hip_global, sizes, 48, 20, seed=14)
+    assert lengths == [512, 1024, 2048, 4096, 8192, 16384]
