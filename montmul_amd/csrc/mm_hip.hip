@@ -929,6 +929,10 @@ int autotune(State* S, bool v2ok) {
 // global mode (kind log2 n), or the radix-4 step, LDS-resident when its vector fits in 64 KiB.
 // Allocates the HBM buffers that mode needs and the batch's own plan did not. Writes the LDS
 // bytes a workgroup needs.
+// LDS that one workgroup of the mixed launch may take: a CU's 160 KiB for one workgroup of 1,024
+// threads, half of it for two of 512.
+inline uint64_t mix_lds_cap(uint32_t nt) { return nt >= 1024 ? 159u * 1024u : 79u * 1024u; }
+
 int mix_class(State* S, uint32_t nt, MixClass* out, uint64_t* lds, uint32_t* groups) {
     const uint32_t n = S->n, k = S->k;
     const size_t sl = S->slots;
@@ -941,6 +945,14 @@ int mix_class(State* S, uint32_t nt, MixClass* out, uint64_t* lds, uint32_t* gro
         *groups = mix_groups(nt, out->kind);
         *lds = *groups > 1 ? (uint64_t)*groups * mix_group_lds(out->kind)
                            : 24ull * std::min<uint32_t>(n, n > 4096 ? 4096 : 2048) + lds_scratch_bytes(nt);  // v2::Tiling
+        return 0;
+    }
+    if (S->wrap && n <= 256 && (uint64_t)(nt / 64) * V1_GROUP_LDS <= mix_lds_cap(nt)) {
+        // One wave per slot, NT / 64 slots per workgroup, each LDS-resident.
+        out->kind = MIX_V1 | MIX_V1_WRAP | MIX_V1_GROUPS;
+        out->k = S->args();
+        *groups = nt / 64;
+        *lds = (uint64_t)*groups * V1_GROUP_LDS;
         return 0;
     }
     const uint64_t resident = lds_bytes_resident(n, k, nt);

@@ -72,6 +72,24 @@ struct Slots {
 __device__ __forceinline__ uint32_t tid() { return threadIdx.x; }
 __device__ __forceinline__ uint32_t nthreads() { return blockDim.x; }
 
+// Thread geometry of the radix-4 step: the whole workgroup steps one slot, or (slot groups in
+// the mixed-length launch) each wave of 64 threads steps its own slot. Barriers stay
+// workgroup-wide, so every group runs the same sequence of them.
+struct WholeBlock {
+    static constexpr bool grouped = false;
+    static __device__ __forceinline__ uint32_t t() { return threadIdx.x; }
+    static __device__ __forceinline__ uint32_t n() { return blockDim.x; }
+};
+// The lane id (mbcnt over a full exec mask) equals threadIdx.x & 63 for whole-wave groups. In
+// the 1,024-thread mixed-length kernel, whose register pressure spills the work-item ID, the
+// step computed from threadIdx.x & 63 came out wrong (every limb, deterministically) while
+// the lane id gave the right results; the 512-thread build was right either way.
+struct OneWave {
+    static constexpr bool grouped = true;
+    static __device__ __forceinline__ uint32_t t() { return __lane_id(); }
+    static __device__ __forceinline__ uint32_t n() { return 64; }
+};
+
 template <class F> struct FieldOf;
 template <> struct FieldOf<F1> { typedef uint32_t B; };
 template <> struct FieldOf<F2> { typedef uint64_t B; };
@@ -120,7 +138,7 @@ __host__ __device__ __forceinline__ uint32_t dif_pass_q(uint32_t logn, uint32_t 
 __host__ __device__ __forceinline__ uint32_t dit_pass_q(uint32_t logn, uint32_t p) { return 1u << ((logn & 1) + 2 * p); }
 
 // Full DIF over a (length t.n). Ends with a barrier.
-template <class F>
+template <class F, class Geo = WholeBlock>
 __device__ void dif(F* a, const Tab& t, const F* dtw) {
     const uint32_t n = t.n, logn = t.logn, npass = logn / 2;
     const bool plus = t.i_is_plus;
@@ -128,7 +146,7 @@ __device__ void dif(F* a, const Tab& t, const F* dtw) {
     for (uint32_t p = 0; p < npass; p++) {
         const uint32_t q = dif_pass_q(logn, p), lq = logn - 2 - 2 * p;
         __syncthreads();
-        for (uint32_t b = tid(); b < n / 4; b += nthreads()) {
+        for (uint32_t b = Geo::t(); b < n / 4; b += Geo::n()) {
             uint32_t j = b & (q - 1);
             uint32_t i0 = ((b >> lq) << (lq + 2)) + j;
             dif4(a, i0, q, dtw + 3 * (off + j), q == 1, plus);
@@ -137,7 +155,7 @@ __device__ void dif(F* a, const Tab& t, const F* dtw) {
     }
     if (logn & 1) {
         __syncthreads();
-        for (uint32_t b = tid(); b < n / 2; b += nthreads()) {
+        for (uint32_t b = Geo::t(); b < n / 2; b += Geo::n()) {
             F u = a[2 * b], v = a[2 * b + 1];
             a[2 * b] = cadd(u, v);
             a[2 * b + 1] = csub(u, v);
@@ -147,13 +165,13 @@ __device__ void dif(F* a, const Tab& t, const F* dtw) {
 }
 
 // Full DIT over a (length t.n), unscaled. Ends with a barrier.
-template <class F>
+template <class F, class Geo = WholeBlock>
 __device__ void dit(F* a, const Tab& t, const F* itw) {
     const uint32_t n = t.n, logn = t.logn, npass = logn / 2;
     const bool plus = t.i_is_plus;
     if (logn & 1) {
         __syncthreads();
-        for (uint32_t b = tid(); b < n / 2; b += nthreads()) {
+        for (uint32_t b = Geo::t(); b < n / 2; b += Geo::n()) {
             F u = a[2 * b], v = a[2 * b + 1];
             a[2 * b] = cadd(u, v);
             a[2 * b + 1] = csub(u, v);
@@ -163,7 +181,7 @@ __device__ void dit(F* a, const Tab& t, const F* itw) {
     for (uint32_t p = 0; p < npass; p++) {
         const uint32_t q = dit_pass_q(logn, p), lq = (logn & 1) + 2 * p;
         __syncthreads();
-        for (uint32_t b = tid(); b < n / 4; b += nthreads()) {
+        for (uint32_t b = Geo::t(); b < n / 4; b += Geo::n()) {
             uint32_t j = b & (q - 1);
             uint32_t i0 = ((b >> lq) << (lq + 2)) + j;
             dit4(a, i0, q, itw + 3 * (off + j), q == 1, plus);
@@ -175,17 +193,19 @@ __device__ void dit(F* a, const Tab& t, const F* itw) {
 
 // ---- Whole-vector element passes ------------------------------------------------------------
 // a_j = (lim_j, lim_{j+n}) * zeta^j for the first cnt coefficients (zero beyond), both fields.
+template <class Geo = WholeBlock>
 __device__ __forceinline__ void load_weighted(F1* a1, F2* a2, const uint32_t* lim, uint32_t cnt, const Tab& t) {
     const uint32_t n = t.n;
-    for (uint32_t j = tid(); j < n; j += nthreads()) {
+    for (uint32_t j = Geo::t(); j < n; j += Geo::n()) {
         uint32_t re = j < cnt ? lim[j] : 0, im = j + n < cnt ? lim[j + n] : 0;
         a1[j] = cmul(F1{mod1(re), mod1(im)}, t.w1[j]);
         a2[j] = cmul(F2{(uint64_t)re, (uint64_t)im}, t.w2[j]);
     }
 }
 
+template <class Geo = WholeBlock>
 __device__ __forceinline__ void unweight(F1* a1, F2* a2, const Tab& t) {
-    for (uint32_t j = tid(); j < t.n; j += nthreads()) {
+    for (uint32_t j = Geo::t(); j < t.n; j += Geo::n()) {
         a1[j] = cmul(a1[j], t.iw1[j]);
         a2[j] = cmul(a2[j], t.iw2[j]);
     }
@@ -201,9 +221,9 @@ __device__ __forceinline__ void unweight(F1* a1, F2* a2, const Tab& t) {
 // generate/propagate ballots over the wave (64 lanes in one 64-bit add), then over the waves.
 // `sc` is LDS scratch of 2 * (blockDim / 64) + 2 words. E >= 2 keeps the neighbour carry's
 // effect to 0 or 1 for every thread except the one owning coefficient count - 1.
-template <int E, class Coef, class Store>
+template <int E, class Coef, class Store, class Geo = WholeBlock>
 __device__ uint64_t block_carry(Coef coef, uint32_t count, Store store, uint64_t* sc, uint64_t cin) {
-    const uint32_t t = tid(), nt = nthreads(), lane = t & 63, wave = t >> 6, nw = nt >> 6;
+    const uint32_t t = Geo::t(), nt = Geo::n(), lane = t & 63, wave = t >> 6, nw = nt >> 6;
     uint64_t chunk_in = cin;
     uint32_t parity = 0;
     for (uint32_t base = 0; base < count; base += nt * E) {

@@ -113,7 +113,7 @@ __device__ __forceinline__ void steps_v2(const v2::VArgs& A, uint32_t slot, uint
 // empty. A register-transform length whose n / 8 threads fill a fraction of the workgroup runs
 // several slots side by side, one per thread group. No reference counterpart: the reference's
 // batches hold one length each (batch.rs:64-77).
-constexpr uint32_t MIX_V1 = 0x100, MIX_V1_GLOBAL = 0x200, MIX_V1_WRAP = 0x400;
+constexpr uint32_t MIX_V1 = 0x100, MIX_V1_GLOBAL = 0x200, MIX_V1_WRAP = 0x400, MIX_V1_GROUPS = 0x800;
 struct MixClass {
     v2::VArgs v;    // register-transform lengths: kind = log2 n (9 .. 13)
     KArgs k;        // every other length: kind = MIX_V1 | flags
@@ -175,6 +175,53 @@ __device__ __forceinline__ void steps_v2_group(const v2::VArgs& A, const uint32_
         if (act) pos++;
     }
     if (on && (threadIdx.x & (T - 1)) == 0) {
+        S.pos[slot] = pos;
+        if (bad) S.err[slot] = 1;
+    }
+}
+
+// LDS of one slot group of the radix-4 step: its LDS-resident layout at the largest grouped
+// length, n = k = 256 with 64 threads (lds_bytes_resident(256, 256, 64)).
+constexpr uint32_t V1_GROUP_LDS = 24u * 256u + 12u * 256u + 8u * (2u * 1u + 2u);
+
+// NT / 64 slots of one radix-4 length (wraparound, n <= 256) in one workgroup, one wave each:
+// wave g steps slot list[first + g] (g < count) for up to `steps` steps in its own LDS-resident
+// layout. A wave without a slot, or whose slot has finished, steps a scratch copy in its own LDS
+// (w.ls, which the wraparound reduction does not use) and writes no slot's state.
+template <uint32_t G>
+__device__ __forceinline__ void steps_v1_group(const KArgs& A, const uint32_t* list, uint32_t first, uint32_t count,
+                                               uint32_t steps, unsigned char* lds) {
+    const Slots& S = A.s;
+    const uint32_t n = A.t.n, k = S.k;
+    const uint32_t g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const bool on = g < count;
+    const uint32_t slot = list[first + (on ? g : 0)];
+    unsigned char* base = lds + g * V1_GROUP_LDS;
+    Work w;
+    w.a2 = (F2*)base;
+    w.a1 = (F1*)(base + 16 * (size_t)n);
+    w.lt = (uint32_t*)(base + 24 * (size_t)n);
+    w.ls = w.lt + 2 * k;
+    w.sc = (uint64_t*)(base + ((24 * (size_t)n + 12 * (size_t)k + 7) & ~(size_t)7));
+    uint32_t* x = S.x + (size_t)slot * k;
+    const uint32_t* ex = S.exp + (size_t)slot * S.ew;
+    const uint32_t nb = S.nbits[slot];
+    uint32_t pos = S.pos[slot];
+    uint32_t bad = 0;
+    for (uint32_t it = 0; it < steps; it++) {
+        const bool act = on && pos < nb;
+        uint32_t* xs = act ? x : w.ls;
+        const uint32_t bit = act ? (ex[pos >> 5] >> (31 - (pos & 31))) & 1 : 0;
+        const uint32_t c = bit ? S.base : 1;
+        const uint32_t b1 = square<OneWave>(w, xs, k, c, A.t) != 0;
+        const uint32_t b2 = redc<true, OneWave>(w, xs, k, A.t, A.ht, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
+                                                S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
+        if (act) {
+            bad |= b1 | b2;
+            pos++;
+        }
+    }
+    if (on && (threadIdx.x & 63) == 0) {
         S.pos[slot] = pos;
         if (bad) S.err[slot] = 1;
     }

@@ -27,11 +27,12 @@ __host__ __device__ inline uint64_t lds_bytes_resident(uint32_t n, uint32_t k, u
 }
 
 // Squaring: T = c x^2 into w.lt (2k limbs); returns the carry out of 2k limbs.
+template <class Geo = WholeBlock>
 __device__ uint64_t square(const Work& w, const uint32_t* x, uint32_t k, uint32_t c, const Tab& t) {
-    load_weighted(w.a1, w.a2, x, k, t);
-    dif(w.a1, t, t.dtw1);
-    dif(w.a2, t, t.dtw2);
-    for (uint32_t j = tid(); j < t.n; j += nthreads()) {
+    load_weighted<Geo>(w.a1, w.a2, x, k, t);
+    dif<F1, Geo>(w.a1, t, t.dtw1);
+    dif<F2, Geo>(w.a2, t, t.dtw2);
+    for (uint32_t j = Geo::t(); j < t.n; j += Geo::n()) {
         F1 v1 = csqr(w.a1[j]);
         F2 v2 = csqr(w.a2[j]);
         if (c != 1) {
@@ -41,16 +42,17 @@ __device__ uint64_t square(const Work& w, const uint32_t* x, uint32_t k, uint32_
         w.a1[j] = v1;
         w.a2[j] = v2;
     }
-    dit(w.a1, t, t.itw1);
-    dit(w.a2, t, t.itw2);
-    unweight(w.a1, w.a2, t);
+    dit<F1, Geo>(w.a1, t, t.itw1);
+    dit<F2, Geo>(w.a2, t, t.itw2);
+    unweight<Geo>(w.a1, w.a2, t);
     __syncthreads();
-    return block_carry<CARRY_E>(ProductCoef{w.a1, w.a2, t.n, t.crt_inv}, 2 * k, StoreLimbs{w.lt}, w.sc, 0);
+    return block_carry<CARRY_E, ProductCoef, StoreLimbs, Geo>(ProductCoef{w.a1, w.a2, t.n, t.crt_inv}, 2 * k, StoreLimbs{w.lt}, w.sc, 0);
 }
 
 // a <- a (.) b elementwise, b in device memory.
+template <class Geo = WholeBlock>
 __device__ __forceinline__ void pointwise_const(const Work& w, const F1* b1, const F2* b2, uint32_t n) {
-    for (uint32_t j = tid(); j < n; j += nthreads()) {
+    for (uint32_t j = Geo::t(); j < n; j += Geo::n()) {
         w.a1[j] = cmul(w.a1[j], b1[j]);
         w.a2[j] = cmul(w.a2[j], b2[j]);
     }
@@ -116,9 +118,9 @@ struct StoreTop {
 
 // half_pointwise (out of place): out_p = (z_p + conj z_q) A_p + (z_p - conj z_q) B_p with
 // q = partner(p), A at ab[p] and B at ab[h + p]. Ends with a barrier.
-template <class F>
+template <class F, class Geo = WholeBlock>
 __device__ __forceinline__ void half_pointwise(const F* z, F* out, const F* ab, uint32_t h) {
-    for (uint32_t p = tid(); p < h; p += nthreads()) {
+    for (uint32_t p = Geo::t(); p < h; p += Geo::n()) {
         F zp = z[p], zq = cconj(z[partner(p)]);
         F e = cadd(zp, zq), d = csub(zp, zq);
         out[p] = cadd(cmul(e, ab[p]), cmul(d, ab[h + p]));
@@ -128,39 +130,44 @@ __device__ __forceinline__ void half_pointwise(const F* z, F* out, const F* ab, 
 
 // Reduction (steps 2 and 3): from T in w.lt, writes T R^-1 (lazy) to x. Returns nonzero when
 // a carry that must vanish did not.
-template <bool WRAP>
+template <bool WRAP, class Geo = WholeBlock>
 __device__ uint32_t redc(const Work& w, uint32_t* x, uint32_t k, const Tab& t, const HalfTab& ht,
                          const F1* nphat1, const F2* nphat2, const F1* nhat1, const F2* nhat2) {
+    // Slot groups share the workgroup's barriers: the reduction without wraparound ORs a flag
+    // over the whole workgroup, so it runs on whole workgroups only.
+    static_assert(WRAP || !Geo::grouped, "slot groups need the wraparound reduction");
     // m = (T mod R) N' mod R
-    load_weighted(w.a1, w.a2, w.lt, k, t);
-    dif(w.a1, t, t.dtw1);
-    dif(w.a2, t, t.dtw2);
-    pointwise_const(w, nphat1, nphat2, t.n);
-    dit(w.a1, t, t.itw1);
-    dit(w.a2, t, t.itw2);
-    unweight(w.a1, w.a2, t);
+    load_weighted<Geo>(w.a1, w.a2, w.lt, k, t);
+    dif<F1, Geo>(w.a1, t, t.dtw1);
+    dif<F2, Geo>(w.a2, t, t.dtw2);
+    pointwise_const<Geo>(w, nphat1, nphat2, t.n);
+    dit<F1, Geo>(w.a1, t, t.itw1);
+    dit<F2, Geo>(w.a2, t, t.itw2);
+    unweight<Geo>(w.a1, w.a2, t);
     __syncthreads();
     if (WRAP) {
         // m goes straight into the packed half-length vector (every read of the product
         // precedes the carry's first barrier; every packed write follows its last).
-        block_carry<CARRY_E>(ProductCoef{w.a1, w.a2, t.n, t.crt_inv}, k, StorePacked{w.a1, w.a2}, w.sc, 0);
+        block_carry<CARRY_E, ProductCoef, StorePacked, Geo>(ProductCoef{w.a1, w.a2, t.n, t.crt_inv}, k, StorePacked{w.a1, w.a2},
+                                                            w.sc, 0);
         // The packed spectrum occupies elements [0, h), the product goes to [h, 2h = n).
         const uint32_t h = ht.t.n;
-        dif(w.a1, ht.t, ht.t.dtw1);
-        dif(w.a2, ht.t, ht.t.dtw2);
+        dif<F1, Geo>(w.a1, ht.t, ht.t.dtw1);
+        dif<F2, Geo>(w.a2, ht.t, ht.t.dtw2);
         F1* b1 = w.a1 + h;
         F2* b2 = w.a2 + h;
-        half_pointwise(w.a1, b1, nhat1, h);
-        half_pointwise(w.a2, b2, nhat2, h);
-        dit(b1, ht.t, ht.t.itw1);
-        dit(b2, ht.t, ht.t.itw2);
+        half_pointwise<F1, Geo>(w.a1, b1, nhat1, h);
+        half_pointwise<F2, Geo>(w.a2, b2, nhat2, h);
+        dit<F1, Geo>(b1, ht.t, ht.t.itw1);
+        dit<F2, Geo>(b2, ht.t, ht.t.itw2);
         // G = (cyclic + T mod R) mod (R - 1), then x = floor(T / R) + G.
-        uint64_t c = block_carry<CARRY_E>(CyclicCoef{b1, b2, w.lt, t.crt_inv}, k, StoreLimbs{x}, w.sc, 0);
+        uint64_t c = block_carry<CARRY_E, CyclicCoef, StoreLimbs, Geo>(CyclicCoef{b1, b2, w.lt, t.crt_inv}, k, StoreLimbs{x}, w.sc, 0);
         // End-around carry: 2^(32k) = 1 mod R - 1. One fold in practice; bounded by the
-        // value's size in principle.
-        for (int guard = 0; c != 0 && guard < 4; guard++)
-            c = block_carry<CARRY_E>(LimbsCoef{x}, k, StoreLimbs{x}, w.sc, c);
-        uint64_t top = block_carry<CARRY_E>(SumCoef{x, w.lt + k}, k, StoreLimbs{x}, w.sc, 0);
+        // value's size in principle. Slot groups fold while any group has a carry (the carry
+        // holds barriers); a fold with carry 0 leaves the limbs as they are.
+        for (int guard = 0; guard < 4 && (Geo::grouped ? __syncthreads_or(c != 0) != 0 : c != 0); guard++)
+            c = block_carry<CARRY_E, LimbsCoef, StoreLimbs, Geo>(LimbsCoef{x}, k, StoreLimbs{x}, w.sc, c);
+        uint64_t top = block_carry<CARRY_E, SumCoef, StoreLimbs, Geo>(SumCoef{x, w.lt + k}, k, StoreLimbs{x}, w.sc, 0);
         return (top != 0) | (c != 0);
     } else {
         bool nz = false;

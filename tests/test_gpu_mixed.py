@@ -83,3 +83,11 @@ def test_mixed_full_slot_groups(hip_global):
     sizes = [6000] * 17 + [12000] * 9 + [20000] * 10 + [40000] * 5 + [100000] * 5 + [150000]
     lengths = common.mixed_lengths(lambda: hip_global, sizes, 48, 20, seed=14)
     assert lengths == [512, 1024, 2048, 4096, 8192, 16384]
+
+
+def test_mixed_radix4_in_1024_thread_launch(hip_global):
+    """With L = 16,384 in the group the launch has 1,024-thread workgroups: the radix-4 step runs
+    there both as one-wave slot groups (L <= 512) and over the whole workgroup in global mode
+    (L = 32,768)."""
+    lengths = common.mixed_lengths(lambda: hip_global, [300000, 150000, 3000, 3000, 7000, 60, 900], 24, 9, seed=15)
+    assert lengths == [8, 64, 256, 512, 16384, 32768]
