@@ -52,7 +52,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_
             case MIX_V1 | MIX_V1_WRAP: steps_v1<false, true, false>(M.k, slot, it.steps, lds); break;
             case MIX_V1 | MIX_V1_GLOBAL: steps_v1<true, false, false>(M.k, slot, it.steps, lds); break;
             case MIX_V1 | MIX_V1_GLOBAL | MIX_V1_WRAP: steps_v1<true, true, false>(M.k, slot, it.steps, lds); break;
-            case MIX_V1 | MIX_V1_WRAP | MIX_V1_GROUPS: steps_v1_group<NT / 64>(M.k, list, it.first, it.count, it.steps, lds); break;
+            case MIX_V1 | MIX_V1_WRAP | MIX_V1_GROUPS: steps_v1_group(M.k, list, it.first, it.count, it.steps, lds); break;
             default: break;
         }
     }

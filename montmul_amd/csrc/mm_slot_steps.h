@@ -184,11 +184,10 @@ __device__ __forceinline__ void steps_v2_group(const v2::VArgs& A, const uint32_
 // length, n = k = 256 with 64 threads (lds_bytes_resident(256, 256, 64)).
 constexpr uint32_t V1_GROUP_LDS = 24u * 256u + 12u * 256u + 8u * (2u * 1u + 2u);
 
-// NT / 64 slots of one radix-4 length (wraparound, n <= 256) in one workgroup, one wave each:
+// blockDim / 64 slots of one radix-4 length (wraparound, n <= 256) in one workgroup, one wave each:
 // wave g steps slot list[first + g] (g < count) for up to `steps` steps in its own LDS-resident
 // layout. A wave without a slot, or whose slot has finished, steps a scratch copy in its own LDS
 // (w.ls, which the wraparound reduction does not use) and writes no slot's state.
-template <uint32_t G>
 __device__ __forceinline__ void steps_v1_group(const KArgs& A, const uint32_t* list, uint32_t first, uint32_t count,
                                                uint32_t steps, unsigned char* lds) {
     const Slots& S = A.s;
