@@ -19,6 +19,9 @@
  *    "... slot i", gpu/mod.rs:179-184), or MM_EFAULT (-2) on any other failure.
  *  - Every call that touches the device makes the batch's device current on the calling
  *    thread first (HIP's current device is per thread).
+ *  - A backend context (mm_hip_open) has one stream, shared by all of its batches: drive a
+ *    context from one thread at a time. Separate contexts, on one device or several, may be
+ *    used from separate threads concurrently.
  *  - Limbs are 32-bit little-endian: limb q holds bits 32q .. 32q+31.
  *  - Exponents are most significant bit first: bit p is (words[p / 32] >> (31 - p % 32)) & 1
  *    (transform.rs:67).
