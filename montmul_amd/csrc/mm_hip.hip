@@ -529,6 +529,12 @@ struct State {
     F2 *d_nphat2 = nullptr, *d_nhat2 = nullptr, *d_work2 = nullptr;
     F1* d_iwc1 = nullptr;  // unweighting times base (register-transform step)
     v2::S2* d_iwc2 = nullptr;
+    // set_modulus staging (N' and N limbs, the half factors' work vectors), allocated on the
+    // first admission and kept: no allocation or free (which can wait on the whole device)
+    // per admitted slot.
+    uint32_t* d_stage_lim = nullptr;
+    F1* d_stage1 = nullptr;
+    F2* d_stage2 = nullptr;
     uint32_t ew = 0;
     // host mirrors
     std::vector<uint32_t> h_x;
@@ -548,7 +554,7 @@ struct State {
         hipSetDevice(C->device);
         for (void* p : {(void*)d_x, (void*)d_exp, (void*)d_nbits, (void*)d_pos, (void*)d_err, (void*)d_lim,
                         (void*)d_nphat1, (void*)d_nhat1, (void*)d_work1, (void*)d_nphat2, (void*)d_nhat2, (void*)d_work2,
-                        (void*)d_iwc1, (void*)d_iwc2})
+                        (void*)d_iwc1, (void*)d_iwc2, (void*)d_stage_lim, (void*)d_stage1, (void*)d_stage2})
             if (p) hipFree(p);
     }
     v2::VArgs vargs() const {
@@ -676,7 +682,9 @@ int push(State* S) {
         uint32_t ew = S->ew;
         for (uint32_t i : S->e_dirty_list) ew = std::max<uint32_t>(ew, (uint32_t)S->exps[i].size());
         if (ew > S->ew || !S->d_exp) {
-            // Longer exponents: a wider buffer, every row uploaded again.
+            // Longer exponents: a wider buffer, every row uploaded again. At least k words, so
+            // that exponents below the modulus (N - 1 for Fermat tests) never grow it again.
+            ew = std::max(ew, S->k);
             HIPCHK(hipStreamSynchronize(sm));
             if (S->d_exp) hipFree(S->d_exp);
             S->d_exp = nullptr;
@@ -1306,11 +1314,11 @@ int mm_set_modulus(void* st, size_t i, const uint32_t* nl, const uint32_t* npl, 
     if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
     const uint32_t k = S->k, n = S->n;
     hipStream_t sm = C->stream;
-    uint32_t* lim = nullptr;
-    F1* t1 = nullptr;
-    F2* t2 = nullptr;
-    if (dalloc(&lim, 2ull * k)) return 1;
-    std::unique_ptr<uint32_t, void (*)(uint32_t*)> g0(lim, [](uint32_t* p) { hipFree(p); });
+    if (!S->d_stage_lim && dalloc(&S->d_stage_lim, 2ull * k)) return 1;
+    if (S->wrap && ((!S->d_stage1 && dalloc(&S->d_stage1, n)) || (!S->d_stage2 && dalloc(&S->d_stage2, n)))) return 1;
+    uint32_t* lim = S->d_stage_lim;
+    F1* t1 = S->d_stage1;
+    F2* t2 = S->d_stage2;
     HIPCHK(hipMemcpyAsync(lim, npl, (size_t)k * 4, hipMemcpyHostToDevice, sm));
     HIPCHK(hipMemcpyAsync(lim + k, nl, (size_t)k * 4, hipMemcpyHostToDevice, sm));
     const uint32_t tt = plan_threads(n);
@@ -1320,7 +1328,6 @@ int mm_set_modulus(void* st, size_t i, const uint32_t* nl, const uint32_t* npl, 
         const uint32_t h = n / 2;
         F1 c1 = cinv(F1{mod1(8 * h), 0});
         F2 c2 = cinv(F2{(uint64_t)(8 * h), 0});
-        if (dalloc(&t1, n) || dalloc(&t2, n)) { hipFree(t1); return 1; }
         hipLaunchKernelGGL(k_half_factors, dim3(1), dim3(plan_threads(h)), 0, sm, S->half->ht, lim + k, k, t1, t2,
                            S->d_nhat1 + i * n, S->d_nhat2 + i * n, c1, c2);
     } else {
@@ -1328,9 +1335,7 @@ int mm_set_modulus(void* st, size_t i, const uint32_t* nl, const uint32_t* npl, 
                            S->d_nhat2 + i * n);
     }
     hipError_t le = hipGetLastError();
-    hipError_t se = hipStreamSynchronize(sm);
-    hipFree(t1);
-    hipFree(t2);
+    hipError_t se = hipStreamSynchronize(sm);  // the staging buffers are reused by the next call
     if (le != hipSuccess) return hip_fail("set_modulus launch", le);
     if (se != hipSuccess) return hip_fail("set_modulus", se);
     S->exps[i].clear();
