@@ -133,6 +133,15 @@ def test_tribonacci_prps(hip):
     common.known_prps(lambda: hip, [4202, 18699], [4203, 18700])
 
 
+def test_tribonacci_prp_96879(hip):
+    """T(96879), 85,169 bits, is a known PRP (tests/gpu.rs:148-153, where it is ignored for its
+    run time): one slot, 85,168 sequential steps at L = 8,192."""
+    t = common.tribonacci(96879)
+    say("T(96879) bits", t.bit_length())
+    assert t.bit_length() == 85_169
+    common.known_prps(lambda: hip, [96879], [])
+
+
 def carry_overflow(make, bits):
     """x = R - 1 is outside [0, 2N); T = 4 (R - 1)^2 carries out of 2k limbs and the error names
     slot 1 (tests/gpu.rs:269-305)."""
