@@ -82,6 +82,11 @@ typedef struct mm_backend {
      * no joint run (the host then runs the states one after another). No reference
      * counterpart: the reference's batches hold one length each (batch.rs:64-77). */
     int64_t (*run_group)(void *const *states, size_t count, uint64_t bits, size_t *which);
+    /* Every slot whose carry check failed in the state's latest run (or mul), ascending: writes
+     * up to `max` of them and returns their count (0 after a clean run). run returns the first
+     * of them; the reference panics on the first (gpu/mod.rs:179-184). NULL when the backend
+     * reports only the first. */
+    size_t (*failed_slots)(void *state, uint32_t *out, size_t max);
     /* Message of the calling thread's last error of this backend. */
     const char *(*last_error)(void);
     /* Limits::coefficient_bound (transform.rs:13), as a 128-bit value. */
@@ -163,6 +168,12 @@ int64_t mm_run_group(void *const *states, size_t count, uint64_t bits, size_t *w
 /* Device time of the step kernel launches of the latest run, in milliseconds (HIP events on
  * the batch's stream), and the number of launches. No reference counterpart: measurement. */
 int mm_last_run_kernel_ms(void *state, double *ms, uint32_t *launches);
+/* The squaring's output of slot `slot`'s latest step: T = c x^2 as 2k limbs (c = base on a 1 bit,
+ * else 1), which global mode keeps in HBM; after mm_run_bare, x is the step's input. Fails in
+ * LDS-resident mode. No reference counterpart: it lets tests check the squaring half alone. */
+int mm_read_product(void *state, size_t slot, uint32_t *out);
+/* The Hip table's failed_slots (see mm_backend). */
+size_t mm_failed_slots(void *state, uint32_t *out, size_t max);
 
 /* ------------------------------------------------------------------------------------ */
 /* The transform layer of a Hip table (trait Transform, transform.rs:22-46; Gpu's impl,      */
@@ -236,6 +247,10 @@ int64_t mm_modbatch_run(mm_modbatch *b, uint64_t bits);
 uint64_t mm_modbatch_remaining(mm_modbatch *b, size_t slot);
 /* ModBatch::mul (batch.rs:162-166); slots must share a modulus. Returns as mm_mul. */
 int64_t mm_modbatch_mul(mm_modbatch *b, size_t out, size_t a, size_t c);
+/* Every slot whose carry check failed in the latest run or mul, ascending (the backend's
+ * failed_slots, or the one slot run returned when the table has none): writes up to `max` and
+ * returns the count. No reference counterpart (the reference panics on the first slot). */
+size_t mm_modbatch_failed_slots(mm_modbatch *b, uint32_t *out, size_t max);
 /* ModBatch::read_raw (batch.rs:168-170): k limbs in [0, 2N). */
 int mm_modbatch_read_raw(mm_modbatch *b, size_t slot, uint32_t *out);
 /* ModBatch::read (batch.rs:173-176): k limbs of raw * R^-1 mod N, in [0, N). */
@@ -268,6 +283,7 @@ int mm_mixbatch_load(mm_mixbatch *b, size_t slot, int sign, const uint32_t *limb
 int mm_mixbatch_load_raw(mm_mixbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
 int mm_mixbatch_set_exponent(mm_mixbatch *b, size_t slot, int sign, const uint32_t *limbs, size_t nlimbs);
 int64_t mm_mixbatch_run(mm_mixbatch *b, uint64_t bits);
+size_t mm_mixbatch_failed_slots(mm_mixbatch *b, uint32_t *out, size_t max);
 uint64_t mm_mixbatch_remaining(mm_mixbatch *b, size_t slot);
 int mm_mixbatch_read_raw(mm_mixbatch *b, size_t slot, uint32_t *out);
 int mm_mixbatch_read(mm_mixbatch *b, size_t slot, uint32_t *out);
@@ -345,6 +361,9 @@ int mm_server_new_hip(const mm_server_options *opts, const mm_options *hip, mm_s
 /* Queues a Fermat test of the odd N > 1 given as little-endian limbs. Returns its id (>= 0),
  * or -1 with mm_server_last_error(). */
 int64_t mm_server_submit(mm_server *s, const uint32_t *limbs, size_t nlimbs);
+/* Queues base^e mod N for a given exponent e >= 0 (little-endian limbs) instead of N - 1; its
+ * result's prp is 1 when base^e = 1 mod N, and `bits` is the bit length of e. */
+int64_t mm_server_submit_exp(mm_server *s, const uint32_t *limbs, size_t nlimbs, const uint32_t *e, size_t ne);
 /* Up to `max` completions, waiting up to `wait_s` seconds for the first. Returns the count. */
 size_t mm_server_poll(mm_server *s, mm_server_result *out, size_t max, double wait_s);
 /* The final raw Montgomery limbs of a completed request (when keep_raw), once. */

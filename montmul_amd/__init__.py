@@ -35,11 +35,13 @@ class MontmulError(RuntimeError):
 
 
 class CarryError(MontmulError):
-    """A carry out of a product that must fit; `slot` names the first such slot."""
+    """A carry out of a product that must fit; `slot` names the first such slot and `slots`
+    every slot the failing run (or mul) hit."""
 
-    def __init__(self, msg, slot):
+    def __init__(self, msg, slot, slots=None):
         super().__init__(msg)
         self.slot = slot
+        self.slots = list(slots) if slots else [slot]
 
 
 class BatchSetup(C.Structure):
@@ -59,6 +61,7 @@ _REMAINING = C.CFUNCTYPE(C.c_uint64, C.c_void_p, C.c_size_t)
 _SETMOD = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                       C.POINTER(C.c_uint32))
 _RUNGROUP = C.CFUNCTYPE(C.c_int64, C.POINTER(C.c_void_p), C.c_size_t, C.c_uint64, C.POINTER(C.c_size_t))
+_FAILED = C.CFUNCTYPE(C.c_size_t, C.c_void_p, C.POINTER(C.c_uint32), C.c_size_t)
 _LASTERR = C.CFUNCTYPE(C.c_char_p)
 
 
@@ -67,7 +70,7 @@ class BackendTable(C.Structure):
     _fields_ = [("self", C.c_void_p), ("create", _CREATE), ("destroy", _DESTROY), ("load", _LOAD),
                 ("set_exponent", _SETEXP), ("run", _RUN), ("mul", _MUL), ("read", _READ),
                 ("remaining", _REMAINING), ("set_modulus", _SETMOD), ("run_group", _RUNGROUP),
-                ("last_error", _LASTERR),
+                ("failed_slots", _FAILED), ("last_error", _LASTERR),
                 ("bound_lo", C.c_uint64), ("bound_hi", C.c_uint64),
                 ("min_len", C.c_uint32), ("max_len", C.c_uint32), ("full_k", C.c_int32)]
 
@@ -124,6 +127,7 @@ def _load_lib():
         "mm_max_lds": (C.c_uint64, [vp]),
         "mm_device_name": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
         "mm_last_run_kernel_ms": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+        "mm_read_product": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_params_for_bits": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p]),
         "mm_params_check_exact": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]),
         "mm_modbatch_new": (C.c_int, [C.POINTER(BackendTable), C.c_size_t, u32p, C.POINTER(C.c_size_t),
@@ -138,6 +142,9 @@ def _load_lib():
         "mm_modbatch_run": (C.c_int64, [vp, C.c_uint64]),
         "mm_modbatch_remaining": (C.c_uint64, [vp, C.c_size_t]),
         "mm_modbatch_mul": (C.c_int64, [vp, C.c_size_t, C.c_size_t, C.c_size_t]),
+        "mm_modbatch_failed_slots": (C.c_size_t, [vp, u32p, C.c_size_t]),
+        "mm_mixbatch_failed_slots": (C.c_size_t, [vp, u32p, C.c_size_t]),
+        "mm_failed_slots": (C.c_size_t, [vp, u32p, C.c_size_t]),
         "mm_modbatch_read_raw": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_modbatch_read": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_modbatch_is_one": (C.c_int, [vp, C.c_size_t]),
@@ -147,6 +154,7 @@ def _load_lib():
         "mm_server_new": (C.c_int, [C.POINTER(ServerOptions), _OPEN, _CLOSE, vp, C.POINTER(vp)]),
         "mm_server_new_hip": (C.c_int, [C.POINTER(ServerOptions), C.POINTER(Options), C.POINTER(vp)]),
         "mm_server_submit": (C.c_int64, [vp, u32p, C.c_size_t]),
+        "mm_server_submit_exp": (C.c_int64, [vp, u32p, C.c_size_t, u32p, C.c_size_t]),
         "mm_server_poll": (C.c_size_t, [vp, C.POINTER(ServerResult), C.c_size_t, C.c_double]),
         "mm_server_raw": (C.c_int, [vp, C.c_int64, u32p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "mm_server_stats_get": (C.c_int, [vp, C.POINTER(ServerStats)]),
@@ -324,6 +332,15 @@ class Hip(Backend):
         _check_run(r, lambda: (lib.mm_last_error() or b"").decode())
 
     @staticmethod
+    def read_product(batch, slot):
+        """T = c x^2 of the slot's latest step (global mode), as an integer."""
+        k = batch.params().k
+        buf = (C.c_uint32 * (2 * k))()
+        if lib.mm_read_product(batch.state, slot, buf) != 0:
+            raise _hip_error()
+        return from_limbs(buf)
+
+    @staticmethod
     def push(batch):
         if lib.mm_push(batch.state) != 0:
             raise MontmulError((lib.mm_last_error() or b"").decode())
@@ -453,12 +470,22 @@ def _transform_methods(cls):
 _transform_methods(Hip)
 
 
-def _check_run(r, msg):
+def _check_run(r, msg, failed=None):
+    """Raises for a run's or mul's result: CarryError naming the first failing slot (with every
+    failing slot from `failed()`, when given), MontmulError for any other failure."""
     if r == MM_OK:
         return
     if r >= 0:
-        raise CarryError(msg(), r)
+        text = msg()
+        raise CarryError(text, r, failed() if failed else None)
     raise MontmulError(msg())
+
+
+def _failed_list(fn, handle):
+    cnt = fn(handle, None, 0)
+    buf = (C.c_uint32 * max(1, cnt))()
+    fn(handle, buf, cnt)
+    return [buf[i] for i in range(cnt)]
 
 
 # ---- host API -----------------------------------------------------------------------------
@@ -556,13 +583,18 @@ class ModBatch:
             raise MontmulError(_host_error())
 
     def run(self, bits):
-        _check_run(lib.mm_modbatch_run(self._h, bits), _host_error)
+        _check_run(lib.mm_modbatch_run(self._h, bits), _host_error, self.failed_slots)
+
+    def failed_slots(self):
+        """Every slot whose carry check failed in the latest run or mul (no reference
+        counterpart: the reference panics on the first)."""
+        return _failed_list(lib.mm_modbatch_failed_slots, self._h)
 
     def remaining(self, i):
         return lib.mm_modbatch_remaining(self._h, i)
 
     def mul(self, out, a, b):
-        _check_run(lib.mm_modbatch_mul(self._h, out, a, b), _host_error)
+        _check_run(lib.mm_modbatch_mul(self._h, out, a, b), _host_error, self.failed_slots)
 
     def read_raw(self, i):
         buf = (C.c_uint32 * self._params.k)()
@@ -659,7 +691,11 @@ class MixBatch:
             raise MontmulError(_host_error())
 
     def run(self, bits):
-        _check_run(lib.mm_mixbatch_run(self._h, bits), _host_error)
+        _check_run(lib.mm_mixbatch_run(self._h, bits), _host_error, self.failed_slots)
+
+    def failed_slots(self):
+        """Every slot whose carry check failed in the latest run."""
+        return _failed_list(lib.mm_mixbatch_failed_slots, self._h)
 
     def remaining(self, i):
         return lib.mm_mixbatch_remaining(self._h, i)
@@ -751,9 +787,15 @@ class Server:
         """Device memory of one pool slot at transform length L in the server's accounting."""
         return lib.mm_server_slot_bytes(length)
 
-    def submit(self, n):
+    def submit(self, n, e=None):
+        """Queues a Fermat test of N (base^(N-1) mod N), or base^e mod N for a given e >= 0.
+        Returns the request's id."""
         buf = to_limbs(int(n))
-        i = lib.mm_server_submit(self._h, buf, len(buf))
+        if e is None:
+            i = lib.mm_server_submit(self._h, buf, len(buf))
+        else:
+            ebuf = to_limbs(int(e))
+            i = lib.mm_server_submit_exp(self._h, buf, len(buf), ebuf, len(ebuf))
         if i < 0:
             raise MontmulError((lib.mm_server_last_error(self._h) or b"").decode())
         return i

@@ -518,6 +518,7 @@ int get_tables(Ctx* C, uint32_t n, std::shared_ptr<DevTables>* out) {
 
 struct State {
     Ctx* C = nullptr;
+    int device = 0;  // C's device, kept so that a state can be freed after its context is closed
     uint32_t k = 0, len = 0, n = 0, base = 0, slots = 0;
     bool global = false, wrap = true, v2 = false;
     uint32_t threads = 0;
@@ -535,7 +536,8 @@ struct State {
     uint32_t* d_stage_lim = nullptr;
     F1* d_stage1 = nullptr;
     F2* d_stage2 = nullptr;
-    uint32_t ew = 0;
+    uint32_t ew = 0;                  // exponent row stride in words
+    size_t exp_cap = 0;               // words allocated for the exponent rows
     // host mirrors
     std::vector<uint32_t> h_x;
     bool x_host_valid = true;         // every row of h_x equals the device's x
@@ -550,8 +552,9 @@ struct State {
     bool tuned = false;
     double last_ms = 0;
     uint32_t last_launches = 0;
+    std::vector<uint32_t> failed;     // every slot whose carry check failed in the latest run or mul
     ~State() {
-        hipSetDevice(C->device);
+        hipSetDevice(device);
         for (void* p : {(void*)d_x, (void*)d_exp, (void*)d_nbits, (void*)d_pos, (void*)d_err, (void*)d_lim,
                         (void*)d_nphat1, (void*)d_nhat1, (void*)d_work1, (void*)d_nphat2, (void*)d_nhat2, (void*)d_work2,
                         (void*)d_iwc1, (void*)d_iwc2, (void*)d_stage_lim, (void*)d_stage1, (void*)d_stage2})
@@ -655,40 +658,56 @@ int set_lds_limit(Ctx* C) {
 // Uploads the rows of x and the exponents written on the host since the last upload: runs of
 // consecutive dirty slots go in one copy each, so that admitting a few slots into a large batch
 // moves only their rows (reference: Gpu::push, gpu/mod.rs:161-176, which uploads everything).
+// Calls copy(first, count) for each run of consecutive slots in `list` (sorted in place).
+template <class Copy> int slot_runs(std::vector<uint32_t>& list, Copy&& copy) {
+    std::sort(list.begin(), list.end());
+    for (size_t a = 0; a < list.size();) {
+        size_t b = a + 1;
+        while (b < list.size() && list[b] == list[b - 1] + 1) b++;
+        if (copy(list[a], list[b - 1] + 1 - list[a])) return 1;
+        a = b;
+    }
+    return 0;
+}
+
+// The x part of push: rows of x written on the host since the last upload. Allocates nothing.
+int push_x(State* S) {
+    if (S->x_dirty_list.empty()) return 0;
+    hipStream_t sm = S->C->stream;
+    const uint32_t k = S->k;
+    if (slot_runs(S->x_dirty_list, [&](uint32_t first, uint32_t cnt) -> int {
+            HIPCHK(hipMemcpyAsync(S->d_x + (size_t)first * k, S->h_x.data() + (size_t)first * k, (size_t)cnt * k * 4,
+                                  hipMemcpyHostToDevice, sm));
+            return 0;
+        }))
+        return 1;
+    for (uint32_t i : S->x_dirty_list) S->x_dirty[i] = 0;
+    S->x_dirty_list.clear();
+    return 0;
+}
+
 int push(State* S) {
     hipStream_t sm = S->C->stream;
-    auto runs = [&](std::vector<uint32_t>& list, auto&& copy) -> int {
-        std::sort(list.begin(), list.end());
-        for (size_t a = 0; a < list.size();) {
-            size_t b = a + 1;
-            while (b < list.size() && list[b] == list[b - 1] + 1) b++;
-            if (copy(list[a], list[b - 1] + 1 - list[a])) return 1;
-            a = b;
-        }
-        return 0;
-    };
-    if (!S->x_dirty_list.empty()) {
-        const uint32_t k = S->k;
-        if (runs(S->x_dirty_list, [&](uint32_t first, uint32_t cnt) -> int {
-                HIPCHK(hipMemcpyAsync(S->d_x + (size_t)first * k, S->h_x.data() + (size_t)first * k, (size_t)cnt * k * 4,
-                                      hipMemcpyHostToDevice, sm));
-                return 0;
-            }))
-            return 1;
-        for (uint32_t i : S->x_dirty_list) S->x_dirty[i] = 0;
-        S->x_dirty_list.clear();
-    }
+    auto runs = [&](std::vector<uint32_t>& list, auto&& copy) -> int { return slot_runs(list, copy); };
+    if (push_x(S)) return 1;
     if (!S->e_dirty_list.empty() || S->exp_all) {
+        // Rows are as wide as the longest exponent set so far, so that re-arming a batch with
+        // short exponents uploads only those words. The buffer holds at least k words per slot,
+        // so that exponents below the modulus (N - 1 for Fermat tests) widen the rows without
+        // reallocating; a wider row uploads every row again.
         uint32_t ew = S->ew;
         for (uint32_t i : S->e_dirty_list) ew = std::max<uint32_t>(ew, (uint32_t)S->exps[i].size());
         if (ew > S->ew || !S->d_exp) {
-            // Longer exponents: a wider buffer, every row uploaded again. At least k words, so
-            // that exponents below the modulus (N - 1 for Fermat tests) never grow it again.
-            ew = std::max(ew, S->k);
-            HIPCHK(hipStreamSynchronize(sm));
-            if (S->d_exp) hipFree(S->d_exp);
-            S->d_exp = nullptr;
-            if (dalloc(&S->d_exp, (size_t)S->slots * ew)) return 1;
+            const size_t need = (size_t)S->slots * ew;
+            if (need > S->exp_cap || !S->d_exp) {
+                const size_t cap = (size_t)S->slots * std::max(ew, S->k);
+                HIPCHK(hipStreamSynchronize(sm));
+                if (S->d_exp) hipFree(S->d_exp);
+                S->d_exp = nullptr;
+                S->exp_cap = 0;
+                if (dalloc(&S->d_exp, cap)) return 1;
+                S->exp_cap = cap;
+            }
             S->ew = ew;
             S->exp_all = true;
         }
@@ -728,23 +747,27 @@ int pull_x(State* S) {
     return 0;
 }
 
-// First slot whose error word is set (clearing all), -1 if none, -2 on a device error.
+// First slot whose error word is set, -1 if none, -2 on a device error. Every flagged slot is
+// appended to S->failed (mm_failed_slots) and all error words are cleared: a failure is reported
+// once, for every slot it hit.
 int64_t check_err(State* S, const char* what) {
     std::vector<uint32_t> err(S->slots);
     hipError_t e = hipMemcpyAsync(err.data(), S->d_err, S->slots * 4, hipMemcpyDeviceToHost, S->C->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(S->C->stream);
     if (e != hipSuccess) { hip_fail(what, e); return MM_EFAULT; }
+    const size_t before = S->failed.size();
     for (uint32_t i = 0; i < S->slots; i++)
-        if (err[i]) {
-            if (hipMemsetAsync(S->d_err, 0, S->slots * 4, S->C->stream) != hipSuccess ||
-                hipStreamSynchronize(S->C->stream) != hipSuccess) {
-                fail(std::string(what) + ": clearing error words failed");
-                return MM_EFAULT;
-            }
-            g_err = std::string(what) + ": carry out of a product that must fit, slot " + std::to_string(i);
-            return (int64_t)i;
-        }
-    return MM_OK;
+        if (err[i]) S->failed.push_back(i);
+    if (S->failed.size() == before) return MM_OK;
+    if (hipMemsetAsync(S->d_err, 0, S->slots * 4, S->C->stream) != hipSuccess || hipStreamSynchronize(S->C->stream) != hipSuccess) {
+        fail(std::string(what) + ": clearing error words failed");
+        return MM_EFAULT;
+    }
+    const uint32_t first = S->failed[before];
+    const size_t more = S->failed.size() - before - 1;
+    g_err = std::string(what) + ": carry out of a product that must fit, slot " + std::to_string(first) +
+            (more ? " (and " + std::to_string(more) + " more slot" + (more > 1 ? "s" : "") + ")" : "");
+    return (int64_t)first;
 }
 
 // Runs `reps` launches of one chunk on the context's stream and waits for them; writes their
@@ -797,6 +820,7 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
     uint64_t left = std::min(bits, most);
     S->last_ms = 0;
     S->last_launches = 0;
+    S->failed.clear();
     if (left == 0) return MM_OK;
     if (push(S)) return MM_EFAULT;
     KArgs A = S->args();
@@ -991,6 +1015,7 @@ int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which)
         if (S->v2 && S->n / 8 > nt) nt = S->n / 8;
         G[g]->last_ms = 0;
         G[g]->last_launches = 0;
+        G[g]->failed.clear();
     }
     uint64_t left = std::min(bits, most);
     if (left == 0) return MM_OK;
@@ -1153,6 +1178,7 @@ int mm_hip_open(const mm_options* opts, mm_backend* out) {
     out->mul = mm_mul;
     out->read = mm_read;
     out->remaining = mm_remaining;
+    out->failed_slots = mm_failed_slots;
     out->last_error = mm_last_error;
     // Limits (gpu/mod.rs:272-283): bound Q = P1 * P2, lengths 2^3..2^24, full_k = wraparound.
     u128 Q = (u128)P1 * P2;
@@ -1190,6 +1216,7 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
     if (wrap && k != n) return fail("wraparound needs k = L/2");
     std::unique_ptr<State> S(new State());
     S->C = C;
+    S->device = C->device;
     S->k = k; S->len = st->len; S->n = n; S->base = st->base; S->slots = st->slots; S->wrap = wrap;
     const char* kenv = getenv("MONTMUL_KERNEL");
     const bool v2ok = plan_fits(S.get(), true, true) && !(kenv && strcmp(kenv, "v1") == 0);
@@ -1219,6 +1246,7 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         dalloc(&S->d_nhat2, sl * n) || dalloc(&S->d_exp, sl))
         return 1;
     S->ew = 1;
+    S->exp_cap = sl;
     if ((S->global || tune) && (dalloc(&S->d_work1, sl * n) || dalloc(&S->d_work2, sl * n) || dalloc(&S->d_lim, sl * 3 * k)))
         return 1;
     if (v2ok) {
@@ -1367,6 +1395,7 @@ int64_t mm_mul(void* st, size_t out, size_t a, size_t b) {  // gpu/mod.rs:386-39
     Ctx* C = S->C;
     if (out >= S->slots || a >= S->slots || b >= S->slots) { fail("slot out of range"); return MM_EFAULT; }
     if (hipSetDevice(C->device) != hipSuccess || push(S)) return MM_EFAULT;
+    S->failed.clear();
     const uint32_t n = S->n, k = S->k;
     F1 *mb1 = nullptr, *w1 = nullptr;
     F2 *mb2 = nullptr, *w2 = nullptr;
@@ -1440,6 +1469,23 @@ int mm_device_name(void* self, char* buf, size_t len) {
     if (len) { strncpy(buf, C->name.c_str(), len - 1); buf[len - 1] = 0; }
     return 0;
 }
+// T = c x^2 (2k limbs) of slot i's latest step, where global mode keeps it in HBM.
+int mm_read_product(void* st, size_t i, uint32_t* out) {
+    State* S = (State*)st;
+    if (i >= S->slots) return fail("slot out of range");
+    if (!S->global || !S->d_lim) return fail("the product stays in HBM only in global mode");
+    if (hipSetDevice(S->C->device) != hipSuccess) return fail("hipSetDevice failed");
+    HIPCHK(hipMemcpyAsync(out, S->d_lim + i * 3 * (size_t)S->k, 2ull * S->k * 4, hipMemcpyDeviceToHost, S->C->stream));
+    HIPCHK(hipStreamSynchronize(S->C->stream));
+    return 0;
+}
+
+size_t mm_failed_slots(void* st, uint32_t* out, size_t max) {
+    const State* S = (const State*)st;
+    for (size_t i = 0; i < S->failed.size() && i < max; i++) out[i] = S->failed[i];
+    return S->failed.size();
+}
+
 int mm_last_run_kernel_ms(void* st, double* ms, uint32_t* launches) {
     State* S = (State*)st;
     *ms = S->last_ms;
@@ -1456,11 +1502,13 @@ struct mm_tbuf {
     uint32_t len, count;
     F1* f1;
     F2* f2;
+    int device;  // C's device: a buffer can be freed after its context is closed
 };
 struct mm_tscales {
     Ctx* C;
     uint32_t count;
     uint32_t* s;
+    int device;
 };
 
 namespace {
@@ -1478,7 +1526,7 @@ int new_tbuf(Ctx* C, uint32_t len, uint32_t count, mm_tbuf** out) {
     *out = nullptr;
     if (check_len(C, len)) return 1;
     if (count == 0) return fail("empty transform batch");
-    std::unique_ptr<mm_tbuf> b(new mm_tbuf{C, len, count, nullptr, nullptr});
+    std::unique_ptr<mm_tbuf> b(new mm_tbuf{C, len, count, nullptr, nullptr, C->device});
     const size_t el = (size_t)len / 2 * count;
     if (dalloc(&b->f1, el) || dalloc(&b->f2, el)) {
         hipFree(b->f1);
@@ -1574,7 +1622,7 @@ int mm_t_upload_scales(void* self, const uint32_t* scales, uint32_t count, mm_ts
     Ctx* C = (Ctx*)self;
     *out = nullptr;
     if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
-    std::unique_ptr<mm_tscales> s(new mm_tscales{C, count, nullptr});
+    std::unique_ptr<mm_tscales> s(new mm_tscales{C, count, nullptr, C->device});
     if (dalloc(&s->s, std::max<uint32_t>(count, 1))) return 1;
     if (count) {
         hipError_t e = hipMemcpy(s->s, scales, (size_t)count * 4, hipMemcpyHostToDevice);
@@ -1616,14 +1664,14 @@ int mm_t_pointwise(mm_tbuf* out, const mm_tbuf* a, const mm_tbuf* b, const mm_ts
 
 void mm_t_free(mm_tbuf* b) {
     if (!b) return;
-    hipSetDevice(b->C->device);
+    hipSetDevice(b->device);
     hipFree(b->f1);
     hipFree(b->f2);
     delete b;
 }
 void mm_t_free_scales(mm_tscales* s) {
     if (!s) return;
-    hipSetDevice(s->C->device);
+    hipSetDevice(s->device);
     hipFree(s->s);
     delete s;
 }

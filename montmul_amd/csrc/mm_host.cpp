@@ -73,6 +73,7 @@ struct mm_modbatch {  // batch.rs:53-58
     void* state = nullptr;
     uint32_t k = 0, len = 0;
     std::vector<MmModulus> moduli;
+    std::vector<uint32_t> failed;  // the latest run's or mul's failing slot, for backends without failed_slots
     ~mm_modbatch() { if (state) be.destroy(state); }
 };
 
@@ -258,8 +259,18 @@ int mm_modbatch_set_exponent(mm_modbatch* b, size_t i, int sign, const uint32_t*
 
 int64_t mm_modbatch_run(mm_modbatch* b, uint64_t bits) {  // batch.rs:152-154
     int64_t r = b->be.run(b->state, bits);
+    b->failed.clear();
+    if (r >= 0) b->failed.push_back((uint32_t)r);
     if (r != MM_OK) g_err = b->be.last_error();
     return r;
+}
+
+// Every slot whose carry check failed in the latest run or mul: the backend's own list when it
+// keeps one, else the one slot that run returned.
+size_t mm_modbatch_failed_slots(mm_modbatch* b, uint32_t* out, size_t max) {
+    if (b->be.failed_slots) return b->be.failed_slots(b->state, out, max);
+    for (size_t i = 0; i < b->failed.size() && i < max; i++) out[i] = b->failed[i];
+    return b->failed.size();
 }
 
 uint64_t mm_modbatch_remaining(mm_modbatch* b, size_t i) {  // batch.rs:157-159
@@ -274,6 +285,8 @@ int64_t mm_modbatch_mul(mm_modbatch* b, size_t out, size_t a, size_t c) {  // ba
         return MM_EFAULT;
     }
     int64_t r = b->be.mul(b->state, out, a, c);
+    b->failed.clear();
+    if (r >= 0) b->failed.push_back((uint32_t)r);
     if (r != MM_OK) g_err = b->be.last_error();
     return r;
 }
@@ -405,6 +418,7 @@ int mm_mixbatch_set_exponent(mm_mixbatch* b, size_t i, int sign, const uint32_t*
 // Returns MM_OK, the first failing slot (its index in this batch), or MM_EFAULT.
 int64_t mm_mixbatch_run(mm_mixbatch* b, uint64_t bits) {
     const mm_backend& be = b->groups[0]->be;
+    for (mm_modbatch* g : b->groups) g->failed.clear();
     if (be.run_group && b->groups.size() > 1) {
         std::vector<void*> st;
         for (mm_modbatch* g : b->groups) st.push_back(g->state);
@@ -419,13 +433,35 @@ int64_t mm_mixbatch_run(mm_mixbatch* b, uint64_t bits) {
         g_err = std::string(be.last_error()) + " (slot " + std::to_string(slot) + " of the mixed batch)";
         return (int64_t)slot;
     }
+    // Group after group; every group takes its steps, and the first failing slot is returned.
+    int64_t first = MM_OK;
+    std::string why;
     for (size_t g = 0; g < b->groups.size(); g++) {
         const int64_t r = mm_modbatch_run(b->groups[g], bits);
         if (r == MM_OK) continue;
         if (r < 0 || (size_t)r >= b->members[g].size()) return r < 0 ? r : MM_EFAULT;
-        return (int64_t)b->members[g][(size_t)r];
+        if (first == MM_OK) {
+            first = (int64_t)b->members[g][(size_t)r];
+            why = g_err + " (slot " + std::to_string(first) + " of the mixed batch)";
+        }
     }
-    return MM_OK;
+    if (first != MM_OK) g_err = why;
+    return first;
+}
+
+// Every slot of the mixed batch whose carry check failed in the latest run, ascending.
+size_t mm_mixbatch_failed_slots(mm_mixbatch* b, uint32_t* out, size_t max) {
+    std::vector<uint32_t> all, part;
+    for (size_t g = 0; g < b->groups.size(); g++) {
+        const size_t cnt = mm_modbatch_failed_slots(b->groups[g], nullptr, 0);
+        part.assign(cnt, 0);
+        mm_modbatch_failed_slots(b->groups[g], part.data(), cnt);
+        for (uint32_t j : part)
+            if (j < b->members[g].size()) all.push_back((uint32_t)b->members[g][j]);
+    }
+    std::sort(all.begin(), all.end());
+    for (size_t i = 0; i < all.size() && i < max; i++) out[i] = all[i];
+    return all.size();
 }
 
 uint64_t mm_mixbatch_remaining(mm_mixbatch* b, size_t i) {

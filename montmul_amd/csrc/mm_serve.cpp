@@ -44,8 +44,8 @@ using Clock = std::chrono::steady_clock;
 struct Request {
     int64_t id;
     std::vector<uint32_t> n;       // little-endian limbs of N
-    std::vector<uint32_t> e;       // N - 1
-    uint64_t bits;
+    std::vector<uint32_t> e;       // N - 1 (a Fermat test), or the exponent given at submit
+    uint64_t bits;                 // bits of e: the steps it takes
     double t_submit;
 };
 
@@ -302,10 +302,19 @@ struct Pool {
                 S->stats.busy_ms += ms;
             }
             if (rr != MM_OK) {
-                // a carry error names a slot; any other failure ends every active request
-                const char* why = mm_host_last_error();
+                // A carry error fails every slot it hit (the others run on); any other failure
+                // ends every active request.
+                const std::string why = mm_host_last_error();
+                std::vector<uint8_t> bad(slot_req.size(), rr < 0 ? 1 : 0);
+                if (rr >= 0) {
+                    std::vector<uint32_t> hit(mm_modbatch_failed_slots(batch, nullptr, 0));
+                    mm_modbatch_failed_slots(batch, hit.data(), hit.size());
+                    hit.push_back((uint32_t)rr);
+                    for (uint32_t i : hit)
+                        if (i < bad.size()) bad[i] = 1;
+                }
                 for (size_t i = 0; i < slot_req.size(); i++)
-                    if (slot_req[i] >= 0 && (rr < 0 || (int64_t)i == rr)) finish(i, 0, why);
+                    if (slot_req[i] >= 0 && bad[i]) finish(i, 0, why.c_str());
                 continue;
             }
             for (size_t i = 0; i < slot_req.size(); i++) {
@@ -377,7 +386,8 @@ int mm_server_new_hip(const mm_server_options* o, const mm_options* hip, mm_serv
     return 0;
 }
 
-int64_t mm_server_submit(mm_server* S, const uint32_t* limbs, size_t nl) {
+// Queues base^e mod N: e = N - 1 (a Fermat test) when exp is null, else the given exponent.
+static int64_t submit(mm_server* S, const uint32_t* limbs, size_t nl, const uint32_t* exp, size_t ne) {
     mpz_t n, e;
     mpz_inits(n, e, nullptr);
     mpz_import(n, nl, -1, 4, 0, 0, limbs);
@@ -397,8 +407,11 @@ int64_t mm_server_submit(mm_server* S, const uint32_t* limbs, size_t nl) {
     r.n.assign((bits + 31) / 32, 0);
     size_t cnt = 0;
     mpz_export(r.n.data(), &cnt, -1, 4, 0, 0, n);
-    mpz_sub_ui(e, n, 1);
-    r.bits = mpz_sizeinbase(e, 2);
+    if (exp)
+        mpz_import(e, ne, -1, 4, 0, 0, exp);
+    else
+        mpz_sub_ui(e, n, 1);
+    r.bits = mpz_sgn(e) ? mpz_sizeinbase(e, 2) : 0;
     r.e.assign((r.bits + 31) / 32 + 1, 0);
     mpz_export(r.e.data(), &cnt, -1, 4, 0, 0, e);
     mpz_clears(n, e, nullptr);
@@ -423,6 +436,17 @@ int64_t mm_server_submit(mm_server* S, const uint32_t* limbs, size_t nl) {
     it->second->cv.notify_one();
     S->stats.submitted++;
     return id;
+}
+
+int64_t mm_server_submit(mm_server* S, const uint32_t* limbs, size_t nl) { return submit(S, limbs, nl, nullptr, 0); }
+
+int64_t mm_server_submit_exp(mm_server* S, const uint32_t* limbs, size_t nl, const uint32_t* e, size_t ne) {
+    if (!e && ne) {
+        fail("null exponent");
+        return -1;
+    }
+    static const uint32_t zero = 0;
+    return submit(S, limbs, nl, e ? e : &zero, e ? ne : 1);
 }
 
 size_t mm_server_poll(mm_server* S, mm_server_result* out, size_t max, double wait_s) {

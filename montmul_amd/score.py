@@ -1,5 +1,6 @@
-"""The efficiency score: sustained modular squarings per second on one fixed workload, reported
-only after the results match pow_mod (reference: montmul/src/score.rs).
+"""Squarings-per-second measurement of a backend on a fixed, seeded workload. A figure is returned
+only when sampled slots agree with pow_mod; repeats re-arm every slot, and the median and MAD of
+their rates are reported (reference: montmul/src/score.rs).
 """
 
 import random

@@ -438,6 +438,7 @@ struct CpuState {  // cpu.rs:421-427
     std::shared_ptr<Tables> tab;
     std::unique_ptr<Half> half;
     std::vector<Slot> slots;
+    std::vector<uint32_t> failed;  // every slot whose carry check failed in the latest run or mul
 };
 
 struct Scratch {  // cpu.rs:429-440
@@ -591,13 +592,23 @@ int64_t oc_run(void* st, uint64_t bits) {  // cpu.rs:543-557
             s.pos += 1;
         }
     });
+    // The reference checks these carries with debug assertions (cpu.rs:474-483) and its GPU
+    // backend panics naming the first failing slot (gpu/mod.rs:179-184): run returns that slot,
+    // and every failing slot is kept for oc_failed_slots.
+    S->failed.clear();
     for (size_t i = 0; i < S->slots.size(); i++)
-        if (S->slots[i].err) {
-            for (auto& s : S->slots) s.err = 0;
-            g_err = "run: carry out of a product that must fit, slot " + std::to_string(i);
-            return (int64_t)i;
-        }
+        if (S->slots[i].err) S->failed.push_back((uint32_t)i);
+    for (auto& s : S->slots) s.err = 0;
+    if (!S->failed.empty()) {
+        g_err = "run: carry out of a product that must fit, slot " + std::to_string(S->failed[0]);
+        return (int64_t)S->failed[0];
+    }
     return MM_OK;
+}
+size_t oc_failed_slots(void* st, uint32_t* out, size_t max) {
+    const CpuState* S = (const CpuState*)st;
+    for (size_t i = 0; i < S->failed.size() && i < max; i++) out[i] = S->failed[i];
+    return S->failed.size();
 }
 int64_t oc_mul(void* st, size_t out, size_t a, size_t b) {  // cpu.rs:558-571
     CpuState* S = (CpuState*)st;
@@ -616,7 +627,12 @@ int64_t oc_mul(void* st, size_t out, size_t a, size_t b) {  // cpu.rs:558-571
     std::vector<uint32_t> x(S->k);
     bad |= redc(S->k, tab, S->half.get(), S->slots[out], sc, x.data(), inv);
     S->slots[out].x = x;
-    if (bad) { g_err = "mul: carry out of a product that must fit, slot " + std::to_string(out); return (int64_t)out; }
+    S->failed.clear();
+    if (bad) {
+        S->failed.push_back((uint32_t)out);
+        g_err = "mul: carry out of a product that must fit, slot " + std::to_string(out);
+        return (int64_t)out;
+    }
     return MM_OK;
 }
 int oc_read(void* st, size_t i, uint32_t* out) {  // cpu.rs:572-574

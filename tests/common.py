@@ -299,6 +299,49 @@ def carry_stress(make, max_len):
             assert b.read(i) == want, f"len {length} slot {i}"
         length *= 2
 
+def load_all_ones(b, i):
+    """Writes x = R - 1 (every limb 0xFFFFFFFF, outside [0, 2N)) into slot i through the
+    backend table's own load, bypassing ModBatch's range check."""
+    import ctypes
+
+    k = b.params().k
+    buf = (ctypes.c_uint32 * k)(*([0xFFFFFFFF] * k))
+    assert b.backend.table.load(ctypes.c_void_p(b.state), i, buf) == 0
+
+
+def carry_overflow_slots(make, bits, bad=(1, 3), slots=4):
+    """Several slots carry out in one run: x = R - 1 in the slots `bad` of a base-4 batch, so
+    T = 4 (R - 1)^2 overflows 2k limbs (tests/gpu.rs:269-305 with more than one failing slot).
+    The error names the first, failed_slots() lists all of them, every error word is cleared
+    (a second run on valid values succeeds with no failed slot), and the other slots are right."""
+    from montmul_amd import CarryError
+
+    rng = random.Random(13)
+    moduli = [rand_odd(rng, bits) for _ in range(slots)]
+    b = ModBatch(make(), moduli, 4)
+    for i in range(slots):
+        b.load(i, 1)
+        b.set_exponent(i, 1)
+    for i in bad:
+        load_all_ones(b, i)
+    try:
+        b.run(1)
+        raise AssertionError("no carry error")
+    except CarryError as e:
+        assert e.slot == bad[0] and f"slot {bad[0]}" in str(e)
+        assert e.slots == sorted(bad), e.slots
+    assert b.failed_slots() == sorted(bad)
+    for i in range(slots):
+        if i not in bad:
+            assert b.read(i) == 4 % moduli[i], f"slot {i}"
+    for i in bad:
+        b.load(i, 5)
+        b.set_exponent(i, 1)
+    b.run(1)
+    assert b.failed_slots() == []
+    for i in bad:
+        assert b.read(i) == 25 * 4 % moduli[i]
+
 
 # ---- Transform layer (common/mod.rs:23-91) -------------------------------------------------
 # `t` is an adapter over a transform backend with upload(coeffs, len, count) -> buf,
@@ -480,3 +523,51 @@ def mixed_lengths(make, bit_sizes, ebits, chunk, seed=9):
     assert with_err
     mix.free()
     return lengths
+
+
+def serve_staggered(server, make, plan, base=3, timeout_s=120.0):
+    """Submits plan = [(delay_s, N, e or None)] in order, sleeping delay_s before each submit
+    (polling meanwhile), so that later requests arrive while earlier ones are on the device.
+    e None is a Fermat test (e = N - 1). Checks every result: prp == [base^e = 1 mod N], the value
+    raw R^-1 mod N == base^e mod N (R = 2^(32 k), k = L/2), and the raw limbs equal those of the
+    same request run alone. Also checks that some request was submitted after another had been
+    admitted and before it completed, and was admitted before that one completed (continuous
+    batching: it joined work already on the device). Returns the results in plan order."""
+    import time
+
+    ids, got = [], {}
+    t_end = time.time() + timeout_s
+
+    def drain(wait):
+        for r in server.poll(256, wait):
+            got[r["id"]] = r
+
+    for delay, n, e in plan:
+        t_next = time.time() + delay
+        while time.time() < t_next:
+            drain(min(0.05, max(0.0, t_next - time.time())))
+        ids.append(server.submit(n, e))
+    while len(got) < len(ids) and time.time() < t_end:
+        drain(0.2)
+    assert len(got) == len(ids), f"{len(ids) - len(got)} requests unfinished"
+    out = []
+    for i, (delay, n, e) in zip(ids, plan):
+        r = got[i]
+        ee = n - 1 if e is None else e
+        want = powmod(base, ee, n)
+        assert r["prp"] == int(want == 1), f"request {i}: prp {r['prp']}"
+        assert r["bits"] == ee.bit_length()
+        assert r["submit_s"] <= r["admit_s"] <= r["done_s"]
+        raw = server.raw(i)
+        k = r["len"] // 2
+        assert raw * powmod(1 << (32 * k), -1, n) % n == want, f"request {i}: value"
+        alone = ModBatch(make(), [n], base)
+        alone.load(0, 1)
+        alone.set_exponent(0, ee)
+        alone.run(1 << 30)
+        assert raw == alone.read_raw(0), f"request {i}: raw differs alone and in the mix"
+        out.append(r)
+    joined = [(a, b) for a in out for b in out
+              if a is not b and a["admit_s"] < b["submit_s"] < a["done_s"] and b["admit_s"] < a["done_s"]]
+    assert joined, "no request arrived while another was running"
+    return out

@@ -76,6 +76,8 @@ def _table(prefix, cfg, bound, min_len, max_len, full_k):
     t.read = mm._READ((prefix + "read", olib))
     t.remaining = mm._REMAINING((prefix + "remaining", olib))
     t.set_modulus = mm._SETMOD((prefix + "set_modulus", olib))
+    if hasattr(olib, prefix + "failed_slots"):
+        t.failed_slots = mm._FAILED((prefix + "failed_slots", olib))
     t.last_error = mm._LASTERR(("oc_last_error", olib))
     t.bound_lo, t.bound_hi = bound & (2**64 - 1), bound >> 64
     t.min_len, t.max_len, t.full_k = min_len, max_len, int(full_k)

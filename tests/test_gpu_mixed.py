@@ -91,3 +91,33 @@ def test_mixed_radix4_in_1024_thread_launch(hip_global):
     (L = 32,768)."""
     lengths = common.mixed_lengths(lambda: hip_global, [300000, 150000, 3000, 3000, 7000, 60, 900], 24, 9, seed=15)
     assert lengths == [8, 64, 256, 512, 16384, 32768]
+
+
+def test_mixed_carry_out_lists_every_slot(hip_global):
+    """Slots of two lengths carry out in the same mixed launch: the error names the first, and
+    failed_slots() lists both (by their index in the mixed batch); the other slots are right."""
+    import ctypes
+    import random
+
+    import montmul_amd as mm
+    from montmul_amd import CarryError
+
+    rng = random.Random(40)
+    moduli = [common.rand_odd(rng, b) for b in (1000, 1000, 20000, 20000, 100000)]
+    mix = MixBatch(hip_global, moduli, 4)
+    assert mix.groups() == 3
+    for i in range(len(moduli)):
+        mix.load(i, 1)
+        mix.set_exponent(i, 1)
+    # slot 1 is index 1 of group 0 (L = 64), slot 3 is index 1 of group 1 (L = 2,048)
+    for g, j in ((0, 1), (1, 1)):
+        st = mm.lib.mm_mixbatch_group_state(mix._h, g)
+        k = mix.params(1 if g == 0 else 3).k
+        buf = (ctypes.c_uint32 * k)(*([0xFFFFFFFF] * k))
+        assert hip_global.table.load(ctypes.c_void_p(st), j, buf) == 0
+    with pytest.raises(CarryError) as ei:
+        mix.run(1)
+    assert ei.value.slot == 1 and ei.value.slots == [1, 3]
+    assert mix.failed_slots() == [1, 3]
+    for i in (0, 2, 4):
+        assert mix.read(i) == 4 % moduli[i]

@@ -166,6 +166,14 @@ def test_carry_overflow_names_the_slot(hip, hip_global):
     carry_overflow(lambda: hip_global, 87_914)
 
 
+def test_carry_overflow_in_several_slots(hip, hip_global):
+    """Two (then three) slots carry out in one run: the error names the first, failed_slots()
+    lists every one, and all error words are cleared for the next run."""
+    common.carry_overflow_slots(lambda: hip, 1000)
+    common.carry_overflow_slots(lambda: hip_global, 100_000)
+    common.carry_overflow_slots(lambda: hip_global, 100_000, bad=(0, 2, 3), slots=5)
+
+
 def test_without_wraparound():
     """tests/gpu.rs:332-345 at smaller sizes."""
     h = Hip(wraparound=False)
@@ -358,3 +366,35 @@ def test_set_modulus(hip, hip_global):
         for bits in (60, 1000, 5000, 100_000):
             common.set_modulus_admission(lambda: be, bits)
     common.set_modulus_admission(lambda: Hip(wraparound=False), 1000)
+
+
+@pytest.mark.parametrize("bits", [1000, 20_000, 100_000, 300_000])
+def test_run_bare_is_the_squaring_alone(hip_global, bits):
+    """Gpu::run_bare (gpu/mod.rs:109-111; lib.rs:949-961): each step is the squaring alone, so x
+    stays as loaded, the position advances, and the product left in HBM is T = c x^2 exactly
+    (c = 3 on a 1 bit, 1 on a 0 bit): the radix-8 register kernel (L = 1,024 .. 8,192) and the
+    radix-4 kernel (L = 64 and 32,768) in global mode."""
+    rng = random.Random(bits)
+    moduli = [common.rand_odd(rng, bits) for _ in range(3)]
+    b = ModBatch(hip_global, moduli, 3)
+    xs = [rng.randrange(2 * n) for n in moduli]
+    for i, x in enumerate(xs):
+        b.load_raw(i, x)
+        b.set_exponent(i, 0b101)
+    for step, c in enumerate((3, 1, 3)):
+        Hip.run_bare(b, 1)
+        for i, x in enumerate(xs):
+            assert b.remaining(i) == 2 - step
+            assert b.read_raw(i) == x, f"slot {i}: x changed"
+            assert Hip.read_product(b, i) == c * x * x, f"bits {bits} slot {i} step {step}"
+
+
+def test_overhead_of_the_reduction():
+    """score.rs:138-160 on a reduced score workload (100,000 bits, 256 slots, 64-bit exponents):
+    run over run_bare, the whole step against the squaring alone, is above 1."""
+    from montmul_amd import score
+
+    cfg = score.ScoreConfig(bits=100_000, slots=256, ebits=64, repeats=3)
+    ratio = score.gpu_overhead(lambda: Hip(), cfg)
+    say("overhead run / run_bare", round(ratio, 3))
+    assert 1.2 < ratio < 4.0

@@ -42,3 +42,36 @@ def test_server_memory_budget(hip):
         assert st["releases"] >= 1 and st["hbm_used"] <= budget
     finally:
         srv.close()
+
+
+def test_server_staggered_up_to_l32768(hip):
+    """Config 5's upper lengths with arrivals while other requests run: requests at L = 8,192
+    (T(96879), an 85,169-bit prime, as a full Fermat test, and a 100,000-bit modulus admitted
+    into that pool's running batch), L = 16,384 (150,000 and 200,000 bits) and L = 32,768
+    (270,000 bits), plus a 20,000-bit planted prime. The long lengths run base^e mod N for short
+    exponents (mm_server_submit_exp) so the test stays in seconds. Each result is checked against
+    pow (prp and the value raw R^-1 mod N) and its raw limbs against the request run alone."""
+    import os
+
+    rng = random.Random(33)
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "planted_20000bit.txt")
+    with open(golden) as f:
+        planted = int(f.readline().strip(), 16)
+    plan = [
+        (0.0, common.tribonacci(96879), None),
+        (0.0, common.rand_odd(rng, 270_000), common.rand_exponent(rng, 3000)),
+        (1.0, common.rand_odd(rng, 100_000), common.rand_exponent(rng, 4000)),
+        (0.0, common.rand_odd(rng, 150_000), common.rand_exponent(rng, 4000)),
+        (1.0, common.rand_odd(rng, 200_000), common.rand_exponent(rng, 2000)),
+        (0.0, planted, None),
+        (0.0, common.rand_odd(rng, 5000), None),
+    ]
+    srv = mm.Server(base=3, pool_slots=4, target_ms=20, keep_raw=True)
+    try:
+        got = common.serve_staggered(srv, lambda: hip, plan, timeout_s=240)
+        assert sorted({r["len"] for r in got}) == [512, 2048, 8192, 16384, 32768]
+        assert got[0]["prp"] == 1 and got[5]["prp"] == 1  # the primes
+        st = srv.stats()
+        assert st["failed"] == 0 and st["completed"] == len(plan)
+    finally:
+        srv.close()

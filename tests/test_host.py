@@ -167,3 +167,88 @@ def test_cli_arguments():
             Args("bench", bad)
     with pytest.raises(SystemExit):
         Args("prp", ["--index", "x"]).num("index")
+
+
+def test_several_slots_carry_out_on_oracle():
+    """Two slots carry out in one run: the error names the first and lists both."""
+    common.carry_overflow_slots(lambda: Cpu(wraparound=True), 1000)
+    common.carry_overflow_slots(lambda: Cpu(wraparound=True), 1000, bad=(0, 2, 3), slots=5)
+
+
+class _CorruptFirstRun(mm.Backend):
+    """The oracle's Cpu backend (wraparound) whose first run with every slot of `bad` active
+    starts those slots from x = R - 1, so their carry checks fail in the same run."""
+
+    def __init__(self, bad):
+        import ctypes
+
+        self.inner = Cpu(wraparound=True)
+        self.bad, self.done, self.k = tuple(bad), False, {}
+        t = mm.BackendTable()
+        for name, _ in mm.BackendTable._fields_:
+            setattr(t, name, getattr(self.inner.table, name))
+
+        def create(selfp, setup, state):
+            r = self.inner.table.create(selfp, setup, state)
+            if r == 0:
+                self.k[state[0]] = setup[0].k
+            return r
+
+        def run(state, bits):
+            st = ctypes.c_void_p(state)
+            if not self.done and all(self.inner.table.remaining(st, i) > 0 for i in self.bad):
+                self.done = True
+                k = self.k[state]
+                buf = (ctypes.c_uint32 * k)(*([0xFFFFFFFF] * k))
+                for i in self.bad:
+                    self.inner.table.load(st, i, buf)
+            return self.inner.table.run(st, bits)
+
+        self._cbs = (mm._CREATE(create), mm._RUN(run))
+        t.create, t.run = self._cbs
+        self.table = t
+
+
+def test_server_fails_every_slot_a_carry_hits():
+    """Requests in slots 0 and 2 of one pool carry out in the same round: both fail, and the
+    request in slot 1 still completes with the right answer and raw limbs."""
+    import random
+    import time
+
+    rng = random.Random(15)
+    moduli = [common.rand_odd(rng, 1000) for _ in range(3)]
+    be = _CorruptFirstRun((0, 2))
+    srv = mm.Server(base=3, pool_slots=4, target_ms=5, keep_raw=True, backend_factory=lambda: be)
+    try:
+        ids = [srv.submit(n) for n in moduli]
+        got, t_end = {}, time.time() + 60
+        while len(got) < 3 and time.time() < t_end:
+            for r in srv.poll(16, 0.2):
+                got[r["id"]] = r
+        assert len(got) == 3
+        assert be.done
+        assert [got[i]["prp"] for i in ids] == [-1, int(pow(3, moduli[1] - 1, moduli[1]) == 1), -1]
+        assert srv.stats()["failed"] == 2
+        alone = ModBatch(Cpu(wraparound=True), [moduli[1]], 3)
+        alone.load(0, 1)
+        alone.set_exponent(0, moduli[1] - 1)
+        alone.run(1 << 20)
+        assert srv.raw(ids[1]) == alone.read_raw(0)
+    finally:
+        srv.close()
+
+
+def test_server_staggered_on_oracle():
+    """Requests arriving while others run (the oracle's Cpu backend per pool), with a given
+    exponent for one of them: every result equals pow and its raw limbs a run alone."""
+    import random
+
+    rng = random.Random(16)
+    plan = [(0.0, common.rand_odd(rng, 3000), None), (0.0, common.rand_odd(rng, 1000), common.rand_exponent(rng, 500)),
+            (0.3, common.rand_odd(rng, 3000), None), (0.1, 2**607 - 1, None), (0.0, common.rand_odd(rng, 200), 0)]
+    srv = mm.Server(base=3, pool_slots=4, target_ms=5, keep_raw=True, backend_factory=lambda: Cpu(wraparound=True))
+    try:
+        got = common.serve_staggered(srv, lambda: Cpu(wraparound=True), plan, timeout_s=120)
+        assert got[3]["prp"] == 1 and got[4]["bits"] == 0 and got[4]["prp"] == 1
+    finally:
+        srv.close()
