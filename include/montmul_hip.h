@@ -174,6 +174,14 @@ int mm_last_run_kernel_ms(void *state, double *ms, uint32_t *launches);
 int mm_read_product(void *state, size_t slot, uint32_t *out);
 /* The Hip table's failed_slots (see mm_backend). */
 size_t mm_failed_slots(void *state, uint32_t *out, size_t max);
+/* mm_mul that allocates nothing. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same result and return values as mm_mul; its device scratch (48 n + 12 k bytes
+ * for L = 2n) comes from a fixed 4 MiB array that each GPU holds for the life of the process,
+ * so lengths above 131,072 are rejected with an error (MM_EFAULT). Loaded rows of x are copied
+ * up first; exponents set on the host wait for the next run. Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (from contexts driven by different threads) race. */
+int64_t mm_mul_noalloc(void *state, size_t out, size_t a, size_t b);
 
 /* ------------------------------------------------------------------------------------ */
 /* The transform layer of a Hip table (trait Transform, transform.rs:22-46; Gpu's impl,      */

@@ -127,6 +127,7 @@ def _load_lib():
         "mm_max_lds": (C.c_uint64, [vp]),
         "mm_device_name": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
         "mm_last_run_kernel_ms": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+        "mm_mul_noalloc": (C.c_int64, [vp, C.c_size_t, C.c_size_t, C.c_size_t]),
         "mm_read_product": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_params_for_bits": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p]),
         "mm_params_check_exact": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]),

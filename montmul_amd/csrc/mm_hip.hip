@@ -24,6 +24,14 @@
 #include "../../include/montmul_hip.h"
 #include "mm_slot_steps.h"
 
+// Device scratch of the allocation-free product (mm_mul_noalloc): the spectrum of one factor and
+// the working vector (n elements of both fields each, 24 n bytes) and 3k limbs, carved from this
+// fixed array. Every GPU the library runs on holds its own copy until the process ends. In host
+// code the name stands for a host-side shadow, not a device address: host code uses only the
+// per-device address that hipGetSymbolAddress returns (Ctx::scratch).
+static constexpr size_t MUL_SCRATCH_BYTES = 4u << 20;
+static __device__ __align__(256) unsigned char scratch[MUL_SCRATCH_BYTES];
+
 namespace mm {
 
 // ---- Kernels ---------------------------------------------------------------------------------
@@ -457,7 +465,30 @@ struct Ctx {
     std::mutex mu;
     std::map<uint32_t, std::shared_ptr<DevTables>> tabs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // This device's copy of the fixed product scratch (null, with scratch_err, if its address
+    // could not be taken).
+    unsigned char* scratch = nullptr;
+    std::string scratch_err;
 };
+
+// Device address of the fixed product scratch on `device`, taken once per device (with that
+// device current) and kept: each GPU holds its own copy of the array at its own address.
+int scratch_address(int device, unsigned char** out) {
+    static std::mutex mu;
+    static std::map<int, unsigned char*> addr;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) {
+        *out = it->second;
+        return 0;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail("hipGetSymbolAddress(scratch)", e != hipSuccess ? e : hipErrorInvalidSymbol);
+    addr[device] = (unsigned char*)p;
+    *out = (unsigned char*)p;
+    return 0;
+}
 
 template <class T> int upload_vec(DevTables& D, const std::vector<T>& v, const T** out) {
     *out = nullptr;
@@ -1167,6 +1198,9 @@ int mm_hip_open(const mm_options* opts, mm_backend* out) {
         delete C;
         return 1;
     }
+    // The fixed product scratch's address on this device (taken on the device's first open).
+    // Only mm_mul_noalloc needs it: a failure is reported there.
+    if (scratch_address(C->device, &C->scratch)) C->scratch_err = g_err;
     out->self = C;
     out->set_modulus = mm_set_modulus;
     out->run_group = mm_run_group;
@@ -1414,6 +1448,52 @@ int64_t mm_mul(void* st, size_t out, size_t a, size_t b) {  // gpu/mod.rs:386-39
     for (void* p : {(void*)mb1, (void*)mb2, (void*)w1, (void*)w2, (void*)lim}) if (p) hipFree(p);
     S->x_host_valid = false;
     return r;
+}
+
+// mm_mul with its scratch (the spectrum of x_b, the working vector, 3k limbs) carved from this
+// device's fixed scratch array instead of allocated per call. Allocates nothing: x rows loaded
+// on the host go up by copies into the batch's existing buffer (exponents wait for the next
+// run), and only the output slot's error word is read back. The kernel writes every scratch
+// word before reading it, so nothing needs clearing between calls.
+int64_t mm_mul_noalloc(void* st, size_t out, size_t a, size_t b) {
+    State* S = (State*)st;
+    Ctx* C = S->C;
+    if (out >= S->slots || a >= S->slots || b >= S->slots) { fail("slot out of range"); return MM_EFAULT; }
+    if (!C->scratch) { fail("mul_noalloc: no fixed scratch on this device: " + C->scratch_err); return MM_EFAULT; }
+    const uint32_t n = S->n, k = S->k;
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_mb2 = up(n * sizeof(F1)), o_w1 = o_mb2 + up(n * sizeof(F2)), o_w2 = o_w1 + up(n * sizeof(F1));
+    const size_t o_lim = o_w2 + up(n * sizeof(F2)), need = o_lim + up(12ull * k);
+    if (need > MUL_SCRATCH_BYTES) {
+        fail("mul_noalloc: length " + std::to_string(S->len) + " needs " + std::to_string(need) +
+             " bytes of scratch, the fixed scratch holds " + std::to_string(MUL_SCRATCH_BYTES) + " (lengths up to 131072)");
+        return MM_EFAULT;
+    }
+    if (hipSetDevice(C->device) != hipSuccess) { fail("hipSetDevice failed"); return MM_EFAULT; }
+    if (push_x(S)) return MM_EFAULT;
+    S->failed.clear();
+    unsigned char* base = C->scratch;
+    auto kern = S->wrap ? k_mul<true> : k_mul<false>;
+    hipLaunchKernelGGL(kern, dim3(1), dim3(S->threads), lds_scratch_bytes(S->threads), C->stream, S->args(), (uint32_t)out,
+                       (uint32_t)a, (uint32_t)b, (F1*)base, (F2*)(base + o_mb2), (F1*)(base + o_w1), (F2*)(base + o_w2),
+                       (uint32_t*)(base + o_lim));
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) { hip_fail("mul_noalloc launch", le); return MM_EFAULT; }
+    S->x_host_valid = false;
+    uint32_t err = 0;
+    hipError_t e = hipMemcpyAsync(&err, S->d_err + out, 4, hipMemcpyDeviceToHost, C->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+    if (e != hipSuccess) { hip_fail("mul_noalloc", e); return MM_EFAULT; }
+    if (err) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, C->stream, S->d_err + out, 1u, 0u);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+        if (e != hipSuccess) { hip_fail("mul_noalloc: clearing the error word", e); return MM_EFAULT; }
+        fail("mul_noalloc: carry out of a product that must fit, slot " + std::to_string(out));
+        S->failed.push_back((uint32_t)out);
+        return (int64_t)out;
+    }
+    return MM_OK;
 }
 
 int mm_read(void* st, size_t i, uint32_t* out) {  // gpu/mod.rs:392-394
