@@ -219,6 +219,23 @@ void mm_t_free(mm_tbuf *b);
 void mm_t_free_scales(mm_tscales *s);
 
 /* ------------------------------------------------------------------------------------ */
+/* Several backends as one (reference: Multi, montmul/src/multi.rs:11-156).               */
+/* ------------------------------------------------------------------------------------ */
+
+/* Fills `out` with a table whose batches split their slots into contiguous shards, one per
+ * part (sizes differing by at most one, multi.rs:39-42), created and run on one host thread
+ * per shard. Per-slot calls go to the slot's shard; a mul across shards runs on the host with
+ * GMP (multi.rs:128-138). A shard's failure is relabelled as the reference's panics are
+ * (multi.rs:58-70): "... slot i" becomes "... shard s, slot g" with g the batch's slot, any
+ * other message "shard s (first slot f): ...". The parts (for instance one mm_hip_open per
+ * device, or several on one device) must outlive the table and have equal limits; each part
+ * is driven from one thread at a time. */
+int mm_multi_open(const mm_backend *parts, size_t count, mm_backend *out);
+void mm_multi_close(mm_backend *be);
+size_t mm_multi_parts(const mm_backend *be);
+const char *mm_multi_last_error(void);
+
+/* ------------------------------------------------------------------------------------ */
 /* Layer 2: the host API over any backend table.                                          */
 /* ------------------------------------------------------------------------------------ */
 
@@ -360,7 +377,8 @@ typedef struct mm_server_stats {
 
 /* Device memory of one pool slot at transform length L in the server's accounting: on the HIP
  * backend in global mode, x and the exponent (k words each), N'-hat and A|B (24 L bytes), the
- * work vectors (12 L) and the limb staging (6 L), with k = L/2: 46 L + 16 bytes. */
+ * work vectors (12 L) and the limb staging (6 L), with k = L/2: 46 L + 16 bytes, plus 12 L at
+ * L = 32,768 for the coefficient staging of its register-transform step. */
 uint64_t mm_server_slot_bytes(uint32_t len);
 
 int mm_server_new(const mm_server_options *opts, mm_open_fn open, mm_close_fn close, void *user, mm_server **out);

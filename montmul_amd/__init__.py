@@ -23,7 +23,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MONTMUL_LIB") or os.path.join(_HERE, "libmontmul_hip.so")
 
-__all__ = ["MontmulError", "CarryError", "Backend", "Hip", "ModBatch", "Params", "fermat",
+__all__ = ["MontmulError", "CarryError", "Backend", "Hip", "Multi", "ModBatch", "Params", "fermat",
            "score_workload", "lib", "LIB_PATH", "MM_OK", "MM_EFAULT"]
 
 MM_OK = -1
@@ -128,6 +128,10 @@ def _load_lib():
         "mm_device_name": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
         "mm_last_run_kernel_ms": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
         "mm_mul_noalloc": (C.c_int64, [vp, C.c_size_t, C.c_size_t, C.c_size_t]),
+        "mm_multi_open": (C.c_int, [C.POINTER(BackendTable), C.c_size_t, C.POINTER(BackendTable)]),
+        "mm_multi_close": (None, [C.POINTER(BackendTable)]),
+        "mm_multi_parts": (C.c_size_t, [C.POINTER(BackendTable)]),
+        "mm_multi_last_error": (C.c_char_p, []),
         "mm_read_product": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_params_for_bits": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p]),
         "mm_params_check_exact": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]),
@@ -351,6 +355,33 @@ class Hip(Backend):
         ms, n = C.c_double(), C.c_uint32()
         lib.mm_last_run_kernel_ms(batch.state, C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+
+class Multi(Backend):
+    """Several backends as one (`Multi`, multi.rs:11-156): a batch's slots are split into
+    contiguous shards, one per part, each created and run on its own host thread; a mul across
+    shards runs on the host. Failures name "shard s, slot g" with g the batch's slot. The parts
+    (for example `Hip()` contexts, on one device or one per device) stay alive with it."""
+
+    def __init__(self, parts):
+        self.parts = list(parts)
+        arr = (BackendTable * max(1, len(self.parts)))()
+        for i, p in enumerate(self.parts):
+            arr[i] = p.table
+        self._arr = arr
+        self.table = BackendTable()
+        if lib.mm_multi_open(arr, len(self.parts), C.byref(self.table)) != 0:
+            raise MontmulError((lib.mm_multi_last_error() or b"").decode())
+
+    def close(self):
+        if self.table is not None and self.table.self:
+            lib.mm_multi_close(C.byref(self.table))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _hip_error():

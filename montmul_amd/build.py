@@ -61,6 +61,7 @@ def build(verbose=False, out=LIB, extra=()):
     hip_obj = os.path.join(OUT_DIR, "mm_hip.o")
     host_obj = os.path.join(OUT_DIR, "mm_host.o")
     serve_obj = os.path.join(OUT_DIR, "mm_serve.o")
+    multi_obj = os.path.join(OUT_DIR, "mm_multi.o")
     # The mixed-length kernel is built once per workgroup size, each in an object of its own:
     # the three device compiles run side by side.
     mix_objs = [os.path.join(OUT_DIR, f"mm_mix{nt}.o") for nt in (512, 1024)]
@@ -70,9 +71,11 @@ def build(verbose=False, out=LIB, extra=()):
              [["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{GMP_INCLUDE}", "-c",
                os.path.join(CSRC, "mm_host.cpp"), "-o", host_obj],
               ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{GMP_INCLUDE}", "-c",
-               os.path.join(CSRC, "mm_serve.cpp"), "-o", serve_obj]], verbose)
+               os.path.join(CSRC, "mm_serve.cpp"), "-o", serve_obj],
+              ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{GMP_INCLUDE}", "-c",
+               os.path.join(CSRC, "mm_multi.cpp"), "-o", multi_obj]], verbose)
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, *mix_objs, host_obj, serve_obj,
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, *mix_objs, host_obj, serve_obj, multi_obj,
           GMP_LIB, "-pthread"], verbose)
     os.replace(tmp, out)
     return out

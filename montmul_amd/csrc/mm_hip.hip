@@ -402,6 +402,20 @@ template <class F> std::vector<F> make_v2_twiddles(const v2::Plan& P, bool inver
     return out;
 }
 
+// omega^j (or omega^-j), j < n/2, with omega the n-th root of the transforms of length n: the
+// twiddles of the radix-2 stage that splits a length-n transform into two of n/2.
+template <class F> std::vector<F> split_roots(uint32_t n, bool inverse) {
+    const F omega = cpow(gen<F>(), (u128)1 << (Ord<F>::E - ilog2(n)));
+    const F base = inverse ? cinv(omega) : omega;
+    std::vector<F> out(n / 2);
+    F r = cone<F>();
+    for (uint32_t j = 0; j < n / 2; j++) {
+        out[j] = r;
+        r = cmul(r, base);
+    }
+    return out;
+}
+
 // GF(M61^2) constants in the split form of the register transforms' products.
 inline std::vector<v2::S2> split_all(const std::vector<F2>& v) {
     std::vector<v2::S2> out(v.size());
@@ -537,6 +551,12 @@ int get_tables(Ctx* C, uint32_t n, std::shared_ptr<DevTables>* out) {
             upload_vec(*D, make_v2_twiddles<F1>(V.P, false), &V.tw1) || upload_vec(*D, split_all(make_v2_twiddles<F2>(V.P, false)), &V.tw2) ||
             upload_vec(*D, make_v2_twiddles<F1>(V.P, true), &V.itw1) || upload_vec(*D, split_all(make_v2_twiddles<F2>(V.P, true)), &V.itw2))
             return 1;
+        // n = 2^14: the radix-2 stage in front of the two sub-transforms of n/2.
+        if (n == 16384 && (upload_vec(*D, split_roots<F1>(n, false), &V.sw1) ||
+                           upload_vec(*D, split_all(split_roots<F2>(n, false)), &V.sw2) ||
+                           upload_vec(*D, split_roots<F1>(n, true), &V.isw1) ||
+                           upload_vec(*D, split_all(split_roots<F2>(n, true)), &V.isw2)))
+            return 1;
     }
     D->ht.t = t;
     if (upload_vec(*D, h1.u, &D->ht.u1) || upload_vec(*D, h1.iu, &D->ht.iu1) || upload_vec(*D, h2.u, &D->ht.u2) ||
@@ -560,6 +580,7 @@ struct State {
     F1 *d_nphat1 = nullptr, *d_nhat1 = nullptr, *d_work1 = nullptr;
     F2 *d_nphat2 = nullptr, *d_nhat2 = nullptr, *d_work2 = nullptr;
     F1* d_iwc1 = nullptr;  // unweighting times base (register-transform step)
+    uint32_t* d_coef = nullptr;  // n = 2^14 register step: coefficient staging, 6 n words per slot
     v2::S2* d_iwc2 = nullptr;
     // set_modulus staging (N' and N limbs, the half factors' work vectors), allocated on the
     // first admission and kept: no allocation or free (which can wait on the whole device)
@@ -588,7 +609,7 @@ struct State {
         hipSetDevice(device);
         for (void* p : {(void*)d_x, (void*)d_exp, (void*)d_nbits, (void*)d_pos, (void*)d_err, (void*)d_lim,
                         (void*)d_nphat1, (void*)d_nhat1, (void*)d_work1, (void*)d_nphat2, (void*)d_nhat2, (void*)d_work2,
-                        (void*)d_iwc1, (void*)d_iwc2, (void*)d_stage_lim, (void*)d_stage1, (void*)d_stage2})
+                        (void*)d_iwc1, (void*)d_iwc2, (void*)d_stage_lim, (void*)d_stage1, (void*)d_stage2, (void*)d_coef})
             if (p) hipFree(p);
     }
     v2::VArgs vargs() const {
@@ -599,6 +620,7 @@ struct State {
         A.crt_inv = tab->t.crt_inv;
         A.iwc1 = d_iwc1;
         A.iwc2 = d_iwc2;
+        A.coef = d_coef;
         return A;
     }
     KArgs args() const {
@@ -640,7 +662,10 @@ StepKernelV2 pick_step_v2(uint32_t logn, bool global, bool bare) {
         case 10: return pick_v2<10>(global, bare);
         case 11: return pick_v2<11>(global, bare);
         case 12: return pick_v2<12>(global, bare);
-        default: return pick_v2<13>(global, bare);
+        case 13: return pick_v2<13>(global, bare);
+        // n = 2^14 runs two sub-transforms of n/2 in turn, in global mode only.
+        case 14: return global ? (bare ? k_steps_v2<14, true, true> : k_steps_v2<14, true, false>) : nullptr;
+        default: return nullptr;
     }
 }
 StepKernel pick_step(bool global, bool wrap, bool bare) {
@@ -669,8 +694,9 @@ int set_lds_limit(Ctx* C) {
         }
         dyn_max = std::min(dyn_max, dyn);
     }
-    for (uint32_t i = 0; i < 20; i++) {
+    for (uint32_t i = 0; i < 24; i++) {
         StepKernelV2 k = pick_step_v2(9 + i / 4, (i >> 1) & 1, i & 1);
+        if (!k) continue;
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, (const void*)k));
         size_t dyn = C->max_lds > a.sharedSizeBytes ? C->max_lds - a.sharedSizeBytes : 0;
@@ -906,7 +932,7 @@ void set_plan(State* S, bool v2, bool global) {
     S->v2 = v2;
     S->global = global;
     if (v2) {
-        S->threads = n / 8;
+        S->threads = std::min<uint32_t>(n / 8, 1024);  // n = 2^14: two halves of n/2, n/16 threads
         const uint64_t tile = std::min<uint32_t>(n, n > 4096 ? 4096 : 2048);  // v2::Tiling
         S->lds = global ? 24 * tile + lds_scratch_bytes(S->threads) : v2::v2_lds_resident(n, k, S->threads);
     } else {
@@ -919,7 +945,8 @@ void set_plan(State* S, bool v2, bool global) {
 // n = 512 .. 8,192; an LDS-resident plan needs its bytes within the kernel's dynamic LDS.
 bool plan_fits(const State* S, bool v2, bool global) {
     const Ctx* C = S->C;
-    if (v2 && !(S->wrap && S->n >= 512 && S->n <= 8192)) return false;
+    if (v2 && !(S->wrap && S->n >= 512 && S->n <= 16384)) return false;
+    if (v2 && S->n > 8192 && !global) return false;  // n = 2^14: global mode only
     if (global) return true;
     if (v2) return v2::v2_lds_resident(S->n, S->k, S->n / 8) <= C->max_dyn_lds_v2;
     return lds_bytes_resident(S->n, S->k, plan_threads(S->n)) <= C->max_dyn_lds;
@@ -1001,7 +1028,9 @@ int mix_class(State* S, uint32_t nt, MixClass* out, uint64_t* lds, uint32_t* gro
     const size_t sl = S->slots;
     memset(out, 0, sizeof(*out));
     *groups = 1;
-    if (S->v2) {
+    // The register-transform step joins the mixed launch up to n = 8,192; n = 2^14 (two halves
+    // in turn) runs the radix-4 step there, like longer lengths.
+    if (S->v2 && n <= 8192) {
         if (!S->d_lim && dalloc(&S->d_lim, sl * 3 * k)) return 1;
         out->kind = ilog2(n);
         out->v = S->vargs();
@@ -1043,7 +1072,7 @@ int64_t group_steps(State* const* G, size_t count, uint64_t bits, size_t* which)
     for (size_t g = 0; g < count; g++) {
         const State* S = G[g];
         for (uint32_t i = 0; i < S->slots; i++) most = std::max<uint64_t>(most, S->nbits[i] - S->pos[i]);
-        if (S->v2 && S->n / 8 > nt) nt = S->n / 8;
+        if (S->v2 && S->n <= 8192 && S->n / 8 > nt) nt = S->n / 8;
         G[g]->last_ms = 0;
         G[g]->last_launches = 0;
         G[g]->failed.clear();
@@ -1267,7 +1296,8 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
     const bool tune = C->opts.autotune && !C->opts.force_global;
     // Device memory: x, N'-hat, N-hat or A|B, work vectors and limbs in global mode, exponents
     // of up to k words, per-slot words, tables (about 6n elements per field, twice), a margin.
-    const uint64_t per_slot = 4ull * k + 48ull * n + (S->global || tune ? 24ull * n + 12ull * k : 0) + 16 + 4ull * k;
+    const uint64_t per_slot = 4ull * k + 48ull * n + (S->global || tune ? 24ull * n + 12ull * k : 0) + 16 + 4ull * k +
+                              (v2ok && n > 8192 ? 24ull * n : 0);
     const uint64_t need = (uint64_t)st->slots * per_slot + 300ull * n + 48ull * n * 2 + (64ull << 20);
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) return fail("hipMemGetInfo failed");
@@ -1285,6 +1315,7 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         return 1;
     if (v2ok) {
         if (dalloc(&S->d_iwc1, n) || dalloc(&S->d_iwc2, n)) return 1;
+        if (n > 8192 && dalloc(&S->d_coef, sl * 6 * n)) return 1;
         hipLaunchKernelGGL(k_scale_tab, dim3((n + 255) / 256), dim3(256), 0, C->stream, S->tab->vt.iw1, S->tab->vt.iw2,
                            st->base, n, S->d_iwc1, S->d_iwc2);
         HIPCHK(hipGetLastError());

@@ -357,7 +357,8 @@ void hip_close(void* user, mm_backend* be) {
 
 extern "C" {
 
-uint64_t mm_server_slot_bytes(uint32_t len) { return 46ull * len + 16; }
+// At L = 32,768 the register-transform step also stages its coefficients in HBM (24 n = 12 L).
+uint64_t mm_server_slot_bytes(uint32_t len) { return 46ull * len + 16 + (len == 32768 ? 12ull * len : 0); }
 
 int mm_server_new(const mm_server_options* o, mm_open_fn open, mm_close_fn close, void* user, mm_server** out) {
     *out = nullptr;

@@ -59,10 +59,11 @@ __device__ __forceinline__ void steps_v1(const KArgs& A, uint32_t slot, uint32_t
     }
 }
 
-// The same steps on register-resident transforms (wraparound path, n = 512 .. 8,192): the
+// The same steps on register-resident transforms (wraparound path, n = 512 .. 16,384): the
 // vector lives in registers, exchanges go through LDS; in global mode T and x stay in HBM and
-// the exchanges use an LDS tile of at most 2,048 elements (4,096 at n = 8,192). WIDE: the
-// workgroup has more than n / 8 threads (see v2_square).
+// the exchanges use an LDS tile of at most 2,048 elements (4,096 at n = 8,192 and 16,384, where
+// the vector is two sub-transforms of n/2 in turn). WIDE: the workgroup has more than n / 8
+// threads (see v2_square).
 template <uint32_t LOGN, bool GLOBAL, bool BARE, bool WIDE = false>
 __device__ __forceinline__ void steps_v2(const v2::VArgs& A, uint32_t slot, uint32_t bits, unsigned char* lds) {
     const Slots& S = A.s;
@@ -94,10 +95,24 @@ __device__ __forceinline__ void steps_v2(const v2::VArgs& A, uint32_t slot, uint
         const uint32_t bit = (ex[pos >> 5] >> (31 - (pos & 31))) & 1;
         const F1* iw1 = bit ? A.iwc1 : A.t.iw1;
         const v2::S2* iw2 = bit ? A.iwc2 : A.t.iw2;
-        bad |= v2::v2_square<LOGN, GLOBAL, WIDE>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv);
-        if (!BARE)
-            bad |= v2::v2_redc<LOGN, GLOBAL, WIDE>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
-                                                   S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
+        if constexpr (LOGN == 14) {
+            // Two sub-transforms of n/2 (v2_square_split): the slot's HBM work vector holds the
+            // half waiting its turn, and coefficients are staged in HBM.
+            static_assert(GLOBAL && !WIDE, "n = 2^14 runs in global mode only");
+            v2::Z1* w1 = (v2::Z1*)(S.work1 + (size_t)slot * n);
+            v2::Z2* w2 = (v2::Z2*)(S.work2 + (size_t)slot * n);
+            uint32_t* coef = A.coef + (size_t)slot * 6 * n;
+            bad |= v2::v2_square_split<LOGN - 1>(A, X, Tl, sc, x, w1, w2, coef, iw1, iw2);
+            if (!BARE)
+                bad |= v2::v2_redc_split<LOGN - 1>(A, X, Tl, sc, x, w1, w2, coef, S.nphat1 + (size_t)slot * n,
+                                                   S.nphat2 + (size_t)slot * n, S.nhat1 + (size_t)slot * n,
+                                                   S.nhat2 + (size_t)slot * n);
+        } else {
+            bad |= v2::v2_square<LOGN, GLOBAL, WIDE>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv);
+            if (!BARE)
+                bad |= v2::v2_redc<LOGN, GLOBAL, WIDE>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
+                                                       S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
+        }
     }
     if (tid() == 0) {
         S.pos[slot] = pos;

@@ -398,3 +398,57 @@ def test_overhead_of_the_reduction():
     ratio = score.gpu_overhead(lambda: Hip(), cfg)
     say("overhead run / run_bare", round(ratio, 3))
     assert 1.2 < ratio < 4.0
+
+
+def test_l32768_register_kernel_against_oracles(hip_global):
+    """L = 32,768 (n = 2^14) runs the register-transform step as two sub-transforms of n/2 in
+    turn. Raw limbs after every slot's chain equal the Gmp closed form (k = L/2) on 16 slots x 64
+    steps, and the Cpu restatement on 2 slots x 12 steps; x = 2N - 1 and N = 2^b - 1 edges."""
+    for bits in (262_200, 500_000):
+        b = ModBatch(hip_global, [common.rand_odd(random.Random(bits), bits)], 3)
+        assert b.params().len == 32768
+        plan = Hip.plan_of(b)
+        assert plan["kernel"] == 8 and plan["global"] and plan["threads"] == 1024, plan
+    say("L=32768 vs Gmp, 16 slots x 64 steps")
+    common.assert_same_raw(lambda: hip_global, lambda: Gmp(full_k=True), 300_000, 16, 64)
+    say("L=32768 vs Cpu, 2 slots x 12 steps")
+    common.assert_same_raw(lambda: hip_global, lambda: Cpu(wraparound=True), 400_000, 2, 12)
+    bits = 32 * 16384 - 4  # the top of the bucket: N = 2^b - 1, x = 2N - 1
+    n = (1 << bits) - 1
+    b = ModBatch(hip_global, [n, common.rand_odd(random.Random(7), bits)], 3)
+    g = ModBatch(Gmp(full_k=True), [n, common.rand_odd(random.Random(7), bits)], 3)
+    for bb in (b, g):
+        bb.load_raw(0, 2 * n - 1)
+        bb.load_raw(1, 12345)
+        for i in range(2):
+            bb.set_exponent(i, 0xF0F1)
+        bb.run(16)
+    for i in range(2):
+        assert b.read_raw(i) == g.read_raw(i), f"edge slot {i}"
+
+
+def test_multi_two_contexts_on_one_device():
+    """tests/gpu.rs:347-357: Multi over two HIP contexts (streams) on device 0, one host thread
+    per shard: raw limbs equal to the Cpu restatement for 1, 5 and 16 slots, powers and mul
+    (within and across shards), Fermat tests; then tests/gpu.rs:368-373, the carry-out of
+    slot 1 (slot 0 of shard 1) named "shard 1, slot 1"."""
+    parts = [Hip(), Hip()]
+    try:
+        two = lambda: mm.Multi(parts)
+        for slots in (1, 5, 16):
+            say("multi: 87,914 bits,", slots, "slots")
+            common.assert_same_raw(two, lambda: Cpu(wraparound=True), 87_914, slots, 50)
+        common.assert_same_raw(two, lambda: Cpu(wraparound=True), 1000, 7, 100)
+        common.powmod_and_mul(two, 2000, 9)
+        common.end_to_end(two, [6, 7, 11, 87, 98, 215, 802, 1001, 2000, 4000])
+        rng = random.Random(12)
+        b = ModBatch(two(), [common.rand_odd(rng, 1000) for _ in range(2)], 4)
+        for i in range(2):
+            b.load(i, 1)
+            b.set_exponent(i, 1)
+        common.load_all_ones(b, 1)
+        with pytest.raises(CarryError, match="shard 1, slot 1"):
+            b.run(1)
+    finally:
+        for p in parts:
+            p.close()

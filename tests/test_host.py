@@ -252,3 +252,60 @@ def test_server_staggered_on_oracle():
         assert got[3]["prp"] == 1 and got[4]["bits"] == 0 and got[4]["prp"] == 1
     finally:
         srv.close()
+
+
+def _two_cpu():
+    return mm.Multi([Cpu(wraparound=True), Cpu(wraparound=True)])
+
+
+def test_multi_on_oracle():
+    """Multi over two oracle backends (tests/gpu.rs:347-357 with the oracle for the GPU parts):
+    shards of 1, 5 and 16 slots give the same raw limbs as one backend; chunked powers and mul,
+    within and across shards; Fermat tests on tribonacci numbers."""
+    for slots in (1, 5, 16):
+        common.assert_same_raw(_two_cpu, lambda: Cpu(wraparound=True), 3000, slots, 40)
+    common.powmod_and_mul(_two_cpu, 2000, 9)
+    common.end_to_end(_two_cpu, [6, 7, 11, 87, 98, 215, 802])
+    two = _two_cpu()
+    assert len(two.parts) == 2 and mm.lib.mm_multi_parts(two.table) == 2
+
+
+def test_multi_cross_shard_mul():
+    """A mul whose operands sit on different shards runs on the host (multi.rs:128-138): the
+    result is a b mod N, in [0, N)."""
+    import random
+
+    rng = random.Random(17)
+    n = common.rand_odd(rng, 1500)
+    b = ModBatch(mm.Multi([Cpu(wraparound=True)] * 3), [n] * 6, 3)
+    xs = [rng.randrange(n) for _ in range(6)]
+    for i, x in enumerate(xs):
+        b.load(i, x)
+    b.mul(5, 0, 3)  # shards 0, 1 and 2
+    assert b.read(5) == xs[0] * xs[3] % n
+    b.mul(1, 1, 1)  # within shard 0
+    assert b.read(1) == xs[1] * xs[1] % n
+
+
+def test_multi_carry_names_the_global_slot():
+    """tests/gpu.rs:368-373: slot 1 is slot 0 of shard 1, and the error says "shard 1, slot 1";
+    with two failing slots on different shards every one is listed (global numbers)."""
+    import random
+
+    from montmul_amd import CarryError
+
+    rng = random.Random(12)
+    b = ModBatch(_two_cpu(), [common.rand_odd(rng, 1000) for _ in range(2)], 4)
+    for i in range(2):
+        b.load(i, 1)
+        b.set_exponent(i, 1)
+    common.load_all_ones(b, 1)
+    with pytest.raises(CarryError, match="shard 1, slot 1") as ei:
+        b.run(1)
+    assert ei.value.slot == 1
+    common.carry_overflow_slots(_two_cpu, 1000, bad=(1, 3), slots=4)
+
+
+def test_multi_rejects_parts_with_other_limits():
+    with pytest.raises(MontmulError, match="part 1 has other limits than part 0"):
+        mm.Multi([Cpu(wraparound=True), Cpu()])

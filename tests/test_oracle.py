@@ -88,6 +88,40 @@ def test_tables_roots(n):
             assert gmul((int(tw[2 * t]), int(tw[2 * t + 1])), (int(itw[2 * t]), int(itw[2 * t + 1])), p) == (1, 0)
 
 
+@pytest.mark.parametrize("n", [16, 256, 16384])
+def test_split_stage_identity(n):
+    """The GPU's register step at n = 2^14 runs a length-n DIF as one radix-2 stage and two
+    DIFs of n/2: A_j = a_j + a_{j+n/2}, B_j = (a_j - a_{j+n/2}) omega^j with omega = g^(2^(E -
+    log n)) (the generator g of order 2^E; the host's split_roots), and DIF_n(a) in bit-reversed
+    order is DIF_{n/2}(A) followed by DIF_{n/2}(B). Checked against the oracle's own DIFs of n
+    and n/2, both fields; at n = 16 the oracle's DIF is itself pinned by the brute-force DFT."""
+    rng = random.Random(n)
+    g1, g2 = sylow()
+    h = n // 2
+    bits = n.bit_length() - 1
+    p32 = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint32))
+    p64 = lambda x: x.ctypes.data_as(C.POINTER(C.c_uint64))
+    f1 = np.array([rng.randrange(P1) for _ in range(2 * n)], dtype=np.uint32)
+    f2 = np.array([rng.randrange(P2) for _ in range(2 * n)], dtype=np.uint64)
+    halves = []
+    for g, e, f, p, dt in ((g1, 32, f1, P1, np.uint32), (g2, 62, f2, P2, np.uint64)):
+        omega = gpow(g, 1 << (e - bits), p)
+        a = [(int(f[2 * j]), int(f[2 * j + 1])) for j in range(n)]
+        A, B, w = [], [], (1, 0)
+        for j in range(h):
+            lo, hi = a[j], a[j + h]
+            A.append(((lo[0] + hi[0]) % p, (lo[1] + hi[1]) % p))
+            B.append(gmul(((lo[0] - hi[0]) % p, (lo[1] - hi[1]) % p), w, p))
+            w = gmul(w, omega, p)
+        halves.append([np.array([c for z in v for c in z], dtype=dt) for v in (A, B)])
+    olib.oc_dif(p32(f1), p64(f2), n)
+    for half in range(2):
+        s1, s2 = halves[0][half], halves[1][half]
+        olib.oc_dif(p32(s1), p64(s2), h)
+        assert np.array_equal(s1, f1[half * n:(half + 1) * n]), f"M31 half {half}"
+        assert np.array_equal(s2, f2[half * n:(half + 1) * n]), f"M61 half {half}"
+
+
 @pytest.mark.parametrize("n", [2, 4, 8, 16, 32])
 def test_dif_is_brute_force_dft_in_bit_reversed_order(n):
     rng = random.Random(n)
