@@ -18,6 +18,8 @@
 // only rotations and adds internally. The host asserts both identities when it builds tables.
 #pragma once
 
+#include <type_traits>
+
 #include "mm_kernels.h"
 
 namespace mm {
@@ -445,16 +447,99 @@ __device__ __forceinline__ void exchange(u32 t, bool act, Z1* v1, Z2* v2, const 
     }
 }
 
+// ---- Exchanges inside one wave ------------------------------------------------------------
+// Between two radix-8 passes of stride 64 or less, the 64 threads of a wave hold exactly the 512
+// positions [512 w, 512 w + 512) of wave w before and after (slot_pos of lane t & 63 plus 512 w).
+// Such an exchange needs no workgroup barrier: each wave moves its block through its own 4 KiB
+// of the exchange buffer, in three rounds of one 8-byte word per element (the M61 real parts,
+// the M61 imaginary parts, the M31 pairs), so that no input has to be held beside its output.
+// The wave's LDS instructions execute in order; the compiler is kept from reordering them
+// across a round by a wave-scope fence.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Word index of block position p (< 512, base-8 digits a2 a1 a0) in the wave's region: a1 ^= a2,
+// a0 ^= a1. Each of the block's three layouts (strides 64, 8 and 1) then puts the 32 lanes of a
+// half-wave 8-byte access on distinct bank pairs.
+__device__ __forceinline__ u32 wsw(u32 p) { return p ^ ((p >> 3) & 7) ^ (((p >> 6) & 7) << 3); }
+// True when the exchange from pass layout (ro, lso) to (ri, lsi) stays inside every wave.
+__host__ __device__ constexpr bool wave_local(u32 ro, u32 lso, u32 ri, u32 lsi) { return ro == 8 && ri == 8 && lso <= 6 && lsi <= 6; }
+
+__device__ __forceinline__ void exchange_wave(u32 t, bool act, Z1* v1, Z2* v2, const Xbuf& X, u32 lso, u32 lsi) {
+    if (!act) return;
+    const u32 l = t & 63;
+    u64* R = (u64*)X.z2 + 512 * (t >> 6);
+    u32 qo[8], qi[8];
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        qo[e] = wsw(slot_pos(l, e, 8, lso, 64));
+        qi[e] = wsw(slot_pos(l, e, 8, lsi, 64));
+    }
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) R[qo[e]] = v2[e].re;
+    wave_sync();
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) v2[e].re = R[qi[e]];
+    wave_sync();
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) R[qo[e]] = v2[e].im;
+    wave_sync();
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) v2[e].im = R[qi[e]];
+    wave_sync();
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) R[qo[e]] = ((u64)v1[e].im << 32) | v1[e].re;
+    wave_sync();
+#pragma unroll
+    for (u32 e = 0; e < 8; e++) {
+        const u64 z = R[qi[e]];
+        v1[e] = Z1{(u32)z, (u32)(z >> 32)};
+    }
+    wave_sync();
+}
+
+// One exchange between passes: inside the waves where it can be, else through the tile. A
+// wave-local exchange that follows a tile exchange (or starts a transform whose caller asks for
+// it) waits at a workgroup barrier first: the tile's last reads alias the waves' regions.
+template <u32 LOGN, bool TILED>
+__device__ __forceinline__ void exchange_any(u32 t, bool act, Z1* v1, Z2* v2, const Xbuf& X, u32 ro, u32 lso, u32 ri, u32 lsi,
+                                             bool& synced) {
+    if (wave_local(ro, lso, ri, lsi)) {
+        if (!synced) __syncthreads();
+        synced = true;
+        exchange_wave(t, act, v1, v2, X, lso, lsi);
+    } else {
+        exchange<LOGN, TILED>(t, act, v1, v2, X, ro, lso, ri, lsi);
+        synced = false;
+    }
+}
+
+// Calls f(std::integral_constant<u32, I>) for I = B .. E-1 in order: the passes of a transform
+// as compile-time constants at every length (an unrolled loop over 5 passes is too large for
+// the compiler to unroll at n >= 8,192, which left radix and stride as run-time values).
+template <u32 B, u32 E, class Fn> __device__ __forceinline__ void static_for(Fn&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<u32, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // Full DIF on registers laid out for pass 0; ends laid out for the last pass (positions
 // 8t .. 8t+7). Threads t >= n / 8 hold no elements but join every barrier, and so do all
-// threads when `on` is false (an idle slot group). GT: see slot_tid.
-template <u32 LOGN, bool TILED, u32 GT = 0>
+// threads when `on` is false (an idle slot group). GT: see slot_tid. SYNC: a barrier before a
+// first exchange that is wave-local (false when, since the last barrier, every wave has used
+// only its own region of the exchange buffer).
+template <u32 LOGN, bool TILED, u32 GT = 0, bool SYNC = true>
 __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, const S2* tw2, const Xbuf& X, bool on = true) {
     typedef CPlan<LOGN> P;
+    static_assert(16 * Tiling<LOGN, TILED>::tile >= 64 * P::T, "the wave regions fit the exchange buffer");
     const bool act = on && t < P::T;
-#pragma unroll
-    for (u32 p = 0; p < P::npass; p++) {
-        const u32 r = P::radix(p), ls = P::lstride(p);
+    bool synced = !SYNC;
+    static_for<0, P::npass>([&](auto pc) {
+        constexpr u32 p = decltype(pc)::value;
+        constexpr u32 r = P::radix(p), ls = P::lstride(p);
         t = slot_tid<GT>();
         if (act) {
             bfly<Op1, true>(v1, r);
@@ -465,19 +550,20 @@ __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, c
             twiddle<Op2>(v2, tw2 + P::tw_off(p), t, r, ls, P::T);
             MM_FENCE();
         }
-        if (p + 1 < P::npass) exchange<LOGN, TILED>(t, act, v1, v2, X, r, ls, P::radix(p + 1), P::lstride(p + 1));
-    }
+        if constexpr (p + 1 < P::npass)
+            exchange_any<LOGN, TILED>(t, act, v1, v2, X, r, ls, P::radix(p + 1), P::lstride(p + 1), synced);
+    });
 }
 
-// Full DIT from the last pass's layout back to pass 0's (natural positions).
-template <u32 LOGN, bool TILED, u32 GT = 0>
+// Full DIT from the last pass's layout back to pass 0's (natural positions). SYNC: as dif_regs.
+template <u32 LOGN, bool TILED, u32 GT = 0, bool SYNC = true>
 __device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, const S2* itw2, const Xbuf& X, bool on = true) {
     typedef CPlan<LOGN> P;
     const bool act = on && t < P::T;
-#pragma unroll
-    for (u32 q = 0; q < P::npass; q++) {
-        const u32 p = P::npass - 1 - q;
-        const u32 r = P::radix(p), ls = P::lstride(p);
+    bool synced = !SYNC;
+    static_for<0, P::npass>([&](auto qc) {
+        constexpr u32 p = P::npass - 1 - decltype(qc)::value;
+        constexpr u32 r = P::radix(p), ls = P::lstride(p);
         t = slot_tid<GT>();
         if (act) {
             twiddle<Op1>(v1, itw1 + P::tw_off(p), t, r, ls, P::T);
@@ -488,8 +574,9 @@ __device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, 
             bfly<Op2, false>(v2, r);
             MM_FENCE();
         }
-        if (p > 0) exchange<LOGN, TILED>(t, act, v1, v2, X, r, ls, P::radix(p - 1), P::lstride(p - 1));
-    }
+        if constexpr (p > 0)
+            exchange_any<LOGN, TILED>(t, act, v1, v2, X, r, ls, P::radix(p - 1), P::lstride(p - 1), synced);
+    });
 }
 
 // ---- Carry over registers ------------------------------------------------------------------

@@ -67,7 +67,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         v1[e] = Op1::sqr(v1[e]);
         v2[e] = Op2::sqr(v2[e]);
     }
-    dit_regs<LOGN, TILED, GT>(t, v1, v2, V.itw1, V.itw2, X, on);
+    dit_regs<LOGN, TILED, GT, false>(t, v1, v2, V.itw1, V.itw2, X, on);  // after the DIF's own exchanges
     t = slot_tid<GT>();
     typedef Tiling<LOGN, TILED> G;
     u32 l[16];
@@ -180,7 +180,7 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
             MM_GROUP_FENCE(e);
         }
     }
-    dit_regs<LOGN, TILED, GT>(t, v1, v2, V.itw1, V.itw2, X, on);
+    dit_regs<LOGN, TILED, GT, false>(t, v1, v2, V.itw1, V.itw2, X, on);  // after the DIF's own exchanges
     __syncthreads();
     t = slot_tid<GT>();
     if (MM_ACT(t)) {
@@ -337,7 +337,7 @@ __device__ __forceinline__ void split_half(const VTab& S, const Xbuf& X, Z1* v1,
             MM_GROUP_FENCE(e);
         }
     }
-    dit_regs<LOGS, true, 0>(slot_tid<0>(), v1, v2, S.itw1, S.itw2, X);
+    dit_regs<LOGS, true, 0, false>(slot_tid<0>(), v1, v2, S.itw1, S.itw2, X);  // after the DIF's own exchanges
 }
 
 // The spectrum's product through both halves: A from registers (its result A' to w[n/2, n)),
