@@ -13,7 +13,8 @@ reference's candidates, main.rs:199-216) at 200-bit exponents, unless --slots is
 headline is global mode (transform vectors in HBM, as the workload specifies); LDS mode
 (vectors resident in LDS) is measured and reported beside it. Slot 0, the last slot and two
 seeded random slots must equal 3^e mod N (GMP powm) or no number is reported
-(score.rs:105-115).
+(score.rs:105-115); after the timed passes every headline slot is checked the same way, on
+the host's threads (--no-check-all skips it).
 
 Prints one JSON line (the last line of stdout). Progress goes to stderr.
 """
@@ -145,6 +146,12 @@ def cpu_baseline(bits, threads, gmp_steps=8_000, ntt_steps=1_000, single_steps=2
     return out
 
 
+def host_threads():
+    """Logical cores this process may run on, up to the 16 host threads of a one-GPU share."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, avail))
+
+
 def copy_calibration(nbytes):
     """Four device-to-device copies of `nbytes` (the HIP runtime's blit kernel,
     __amd_rocclr_copyBuffer): a plain copy of known size whose FETCH_SIZE / WRITE_SIZE counters
@@ -177,6 +184,8 @@ def main():
     ap.add_argument("--sweep-ebits", type=int, default=200)
     ap.add_argument("--modes", default="global,lds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check-all", action="store_true",
+                    help="skip the check of every headline slot against GMP's powm (about 3 minutes on 16 threads)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (4 times): the reference kernel for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -206,6 +215,18 @@ def main():
         log(f"mode {mode}: {best} slots, {args.warmup} warm-up + {args.steps} timed passes of run({args.ebits})")
         b, rates, secs, kms, ups = measure(mm, hip, moduli, exps, best, args.ebits, args.steps, args.warmup)
         picked = check(b, exps, 2026)
+        all_slots = None
+        if mode == "global" and not args.no_check_all:
+            # Every slot of the last timed pass against GMP's powm on the host share's threads.
+            t_chk, part = time.perf_counter(), 512
+            for first in range(0, best, part):
+                bad = b.check_pow_range(3, first, exps[first:min(best, first + part)], host_threads())
+                if bad is not None:
+                    raise SystemExit(f"slot {bad} disagrees with pow_mod; no score reported")
+                log(f"  slots {first}..{min(best, first + part) - 1} equal 3^e mod N")
+            all_slots = {"slots": best, "equal_to_powm": True, "seconds": round(time.perf_counter() - t_chk, 1),
+                         "threads": host_threads()}
+            log(f"mode {mode}: all {best} slots equal 3^e mod N ({all_slots['seconds']} s)")
         plan = mm.Hip.plan_of(b)
         L, k = b.params().len, b.params().k
         # run over run_bare (score.rs:138-160) from the step kernel's own device time: the whole
@@ -215,7 +236,8 @@ def main():
         bare_ms, bare_launches = mm.Hip.last_run_kernel_ms(b)
         overhead = statistics.mean(kms) / (bare_ms / max(1, bare_launches))
         results[mode] = dict(slots=best, rates=rates, secs=secs, kernel_ms=kms, plan=plan, L=L, k=k, ups=ups,
-                             sweep={str(s): round(r) for s, r in sw.items()}, checked=picked, overhead=overhead)
+                             sweep={str(s): round(r) for s, r in sw.items()}, checked=picked, overhead=overhead,
+                             all_slots=all_slots)
         log(f"mode {mode}: median {median(rates):.4g} sq/s, kernel {statistics.mean(kms):.1f} ms/launch")
         b.free()
         hip.close()
@@ -264,6 +286,7 @@ def main():
                      "kernel": "k_steps_v2 (global mode)", "kernel_ms_per_launch": round(kms, 3),
                      "alg_bytes_per_slot_step": 28 * L},
         "checked_slots": head["checked"],
+        "checked_all_slots": head.get("all_slots"),
         # The bound that binds: VALU issue (integer work, no MFMA). Instructions per slot-step
         # from the profile, rate from this run's kernel time.
         "valu": ({"bound": "valu", "achieved": round(VALU_PER_SLOT_STEP[L] * slots * ebits / (kms * 1e-3) / 1e9, 2),
@@ -283,7 +306,7 @@ def main():
         # Every logical core this process may run on, up to the 16 host threads that a one-GPU
         # share of the GPU host allows (its worker-pool limit); both counts are reported.
         avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        threads = max(1, min(16, avail))
+        threads = host_threads()
         log(f"cpu baseline on {threads} threads ({avail} logical cores available, {os.cpu_count()} in the host)")
         cb = cpu_baseline(args.bits, threads)
         g, c, g1 = cb["gmp"], cb["cpu_ntt"], cb["gmp_single"]

@@ -218,6 +218,13 @@ int mm_t_pointwise(mm_tbuf *out, const mm_tbuf *a, const mm_tbuf *b, const mm_ts
 void mm_t_free(mm_tbuf *b);
 void mm_t_free_scales(mm_tscales *s);
 
+/* Issue-rate probe (no reference counterpart: measurement). Runs `blocks` workgroups of 256
+ * threads, each thread `iters` iterations of 8 independent chains of one instruction: op 0
+ * v_mad_u64_u32, 1 v_add_u32, 2 v_mul_hi_u32, 3 v_lshl_add_u64 (64-bit add), 4 v_mad_u64_u32 and
+ * v_add_u32 in turn. Writes the launch's device time in ms and a checksum of the results. */
+int mm_probe_issue(int device, int op, uint32_t blocks, uint32_t iters, double *ms, uint64_t *checksum);
+const char *mm_probe_last_error(void);
+
 /* ------------------------------------------------------------------------------------ */
 /* Several backends as one (reference: Multi, montmul/src/multi.rs:11-156).               */
 /* ------------------------------------------------------------------------------------ */
@@ -324,6 +331,13 @@ int mm_fermat(const mm_backend *be, size_t count, const uint32_t *limbs, const s
 /* The score's check of one slot (montmul/src/score.rs:107-115): no exponent bits left and
  * read(slot) == base^e mod N, e given as little-endian limbs. 1 if so, 0 if not, < 0 on error. */
 int mm_modbatch_check_pow(mm_modbatch *b, size_t slot, uint32_t base, const uint32_t *e, size_t ne);
+/* The same check on slots first_slot .. first_slot+count-1 (the j-th one's exponent: limbs
+ * exps[offsets[j] .. offsets[j+1])). The slots are read on the calling thread and GMP's powm
+ * runs on `threads` threads. Returns -1 when every slot matches, else the lowest slot that does
+ * not (or has bits left), or -2 on an error. No reference counterpart: the score checks a few
+ * slots; this checks a whole batch in parts a caller can report progress between. */
+int64_t mm_modbatch_check_pow_range(mm_modbatch *b, uint32_t base, size_t first_slot, size_t count,
+                                    const uint32_t *exps, const size_t *offsets, uint32_t threads);
 
 /* The score workload (montmul/src/score.rs:45-61): `slots` moduli of `bits` bits drawn from
  * GMP's default random state seeded with `seed`, top bit and bit 0 set, and per modulus the top

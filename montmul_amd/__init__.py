@@ -132,6 +132,8 @@ def _load_lib():
         "mm_multi_close": (None, [C.POINTER(BackendTable)]),
         "mm_multi_parts": (C.c_size_t, [C.POINTER(BackendTable)]),
         "mm_multi_last_error": (C.c_char_p, []),
+        "mm_probe_issue": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_double), u64p]),
+        "mm_probe_last_error": (C.c_char_p, []),
         "mm_read_product": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_params_for_bits": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p]),
         "mm_params_check_exact": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]),
@@ -148,6 +150,8 @@ def _load_lib():
         "mm_modbatch_remaining": (C.c_uint64, [vp, C.c_size_t]),
         "mm_modbatch_mul": (C.c_int64, [vp, C.c_size_t, C.c_size_t, C.c_size_t]),
         "mm_modbatch_failed_slots": (C.c_size_t, [vp, u32p, C.c_size_t]),
+        "mm_modbatch_check_pow_range": (C.c_int64, [vp, C.c_uint32, C.c_size_t, C.c_size_t, u32p,
+                                                     C.POINTER(C.c_size_t), C.c_uint32]),
         "mm_mixbatch_failed_slots": (C.c_size_t, [vp, u32p, C.c_size_t]),
         "mm_failed_slots": (C.c_size_t, [vp, u32p, C.c_size_t]),
         "mm_modbatch_read_raw": (C.c_int, [vp, C.c_size_t, u32p]),
@@ -647,6 +651,15 @@ class ModBatch:
         if r < 0:
             raise MontmulError(_host_error())
         return bool(r)
+
+    def check_pow_range(self, base, first, exps, threads=1):
+        """check_pow on slots first .. first+len(exps)-1 (GMP powm on `threads` host threads):
+        None when all match, else the lowest slot that does not."""
+        limbs, offsets, _ = _pack_ints([int(e) for e in exps])
+        r = lib.mm_modbatch_check_pow_range(self._h, base, first, len(exps), limbs, offsets, threads)
+        if r == -2:
+            raise MontmulError(_host_error())
+        return None if r == -1 else r
 
     def is_one(self, i):
         r = lib.mm_modbatch_is_one(self._h, i)

@@ -59,23 +59,25 @@ def _run_all(cmds, verbose):
 def build(verbose=False, out=LIB, extra=()):
     os.makedirs(OUT_DIR, exist_ok=True)
     hip_obj = os.path.join(OUT_DIR, "mm_hip.o")
+    probe_obj = os.path.join(OUT_DIR, "mm_probe.o")
     host_obj = os.path.join(OUT_DIR, "mm_host.o")
     serve_obj = os.path.join(OUT_DIR, "mm_serve.o")
     multi_obj = os.path.join(OUT_DIR, "mm_multi.o")
     # The mixed-length kernel is built once per workgroup size, each in an object of its own:
     # the three device compiles run side by side.
     mix_objs = [os.path.join(OUT_DIR, f"mm_mix{nt}.o") for nt in (512, 1024)]
-    _run_all([[HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, "mm_hip.hip"), "-o", hip_obj]] +
+    _run_all([[HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, "mm_hip.hip"), "-o", hip_obj],
+              [HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, "mm_probe.hip"), "-o", probe_obj]] +
              [[HIPCC, *HIP_FLAGS, *extra, f"-DMM_MIX_NT={nt}", "-c", os.path.join(CSRC, "mm_mix.hip"), "-o", o]
               for nt, o in zip((512, 1024), mix_objs)] +
-             [["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{GMP_INCLUDE}", "-c",
+             [["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{GMP_INCLUDE}", "-c",
                os.path.join(CSRC, "mm_host.cpp"), "-o", host_obj],
               ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{GMP_INCLUDE}", "-c",
                os.path.join(CSRC, "mm_serve.cpp"), "-o", serve_obj],
               ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{GMP_INCLUDE}", "-c",
                os.path.join(CSRC, "mm_multi.cpp"), "-o", multi_obj]], verbose)
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, *mix_objs, host_obj, serve_obj, multi_obj,
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, hip_obj, probe_obj, *mix_objs, host_obj, serve_obj, multi_obj,
           GMP_LIB, "-pthread"], verbose)
     os.replace(tmp, out)
     return out

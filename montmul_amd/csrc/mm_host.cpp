@@ -63,6 +63,46 @@ template <class Fn> void par_for(size_t count, Fn fn) {
 
 typedef unsigned __int128 u128;
 
+// The Montgomery constants of an odd modulus N > 1 for R = 2^(32k) (batch.rs:84-91):
+// N' = -N^-1 mod R, R mod N and R^-1 mod N. The reference inverts twice (N mod R, then R mod N);
+// here neither needs an extended gcd:
+// - N^-1 mod 2^p comes from Newton's iteration x <- x (2 - N x) mod 2^p, which doubles the
+//   count of correct low bits (Hensel lifting): from 64 bits (five rounds on one machine word)
+//   up a ladder of precisions ceil(32k / 2^j), so the last round works at exactly 32k bits.
+// - With N N^-1 = 1 + q R (0 <= q < N), 1 + N N' = (N - q) R, so R^-1 mod N = (1 + N N') / R,
+//   one product and a shift (this is REDC(1)).
+// Returns false when N is even or not below R.
+bool montgomery_constants(const mpz_t n, uint32_t k, mpz_t nprime, mpz_t r_mod_n, mpz_t rinv) {
+    const mp_bitcnt_t bits = 32 * (mp_bitcnt_t)k;
+    if (mpz_sgn(n) <= 0 || mpz_even_p(n) || mpz_sizeinbase(n, 2) > bits) return false;
+    const uint64_t n0 = (uint64_t)mpz_getlimbn(n, 0);
+    uint64_t x0 = n0;  // N N = 1 mod 8: three bits to start
+    for (int i = 0; i < 5; i++) x0 *= 2 - n0 * x0;
+    std::vector<mp_bitcnt_t> ladder;
+    for (mp_bitcnt_t p = bits; p > 64; p = (p + 1) / 2) ladder.push_back(p);
+    Mpz x, t, nlow;
+    mpz_set_ui(x.v, (unsigned long)x0);
+    if (bits < 64) mpz_fdiv_r_2exp(x.v, x.v, bits);
+    for (size_t j = ladder.size(); j-- > 0;) {
+        const mp_bitcnt_t p = ladder[j];
+        mpz_fdiv_r_2exp(nlow.v, n, p);
+        mpz_mul(t.v, nlow.v, x.v);
+        mpz_fdiv_r_2exp(t.v, t.v, p);
+        mpz_ui_sub(t.v, 2, t.v);
+        mpz_mul(x.v, x.v, t.v);
+        mpz_fdiv_r_2exp(x.v, x.v, p);
+    }
+    // nprime = R - x (x is odd, so nonzero)
+    mpz_set_ui(t.v, 0);
+    mpz_setbit(t.v, bits);
+    mpz_sub(nprime, t.v, x.v);
+    mpz_mod(r_mod_n, t.v, n);
+    mpz_mul(rinv, n, nprime);
+    mpz_add_ui(rinv, rinv, 1);
+    mpz_fdiv_q_2exp(rinv, rinv, bits);
+    return true;
+}
+
 }  // namespace
 
 // Modulus (batch.rs:44-50).
@@ -145,16 +185,11 @@ int mm_modbatch_new(const mm_backend* be, size_t count, const uint32_t* limbs, c
     std::atomic<int> bad(0);
     // Two modular inversions per modulus dominate the setup of large batches.
     par_for(count, [&](size_t i) {
-        Mpz r, ninv, np, two;
-        mpz_setbit(r.v, 32 * (mp_bitcnt_t)k);
+        Mpz np, two;
         const mpz_t& n = ns[i].v;
-        if (!mpz_invert(ninv.v, n, r.v)) { bad = 1; return; }
-        mpz_sub(np.v, r.v, ninv.v);
-        mpz_mod(np.v, np.v, r.v);
         MmModulus& m = B->moduli[i];
         mpz_set(m.n.v, n);
-        mpz_mod(m.r.v, r.v, n);
-        if (!mpz_invert(m.rinv.v, r.v, n)) { bad = 1; return; }
+        if (!montgomery_constants(n, k, np.v, m.r.v, m.rinv.v)) { bad = 1; return; }
         mpz_mul_2exp(two.v, n, 1);
         if (!to_limbs(n, &nl[i * k], k) || !to_limbs(np.v, &npl[i * k], k) || !to_limbs(two.v, &tnl[i * k], k))
             bad = 1;
@@ -177,7 +212,7 @@ int mm_modbatch_new(const mm_backend* be, size_t count, const uint32_t* limbs, c
 int mm_modbatch_set_modulus(mm_modbatch* b, size_t i, int sign, const uint32_t* limbs, size_t nl) {
     if (i >= b->moduli.size()) return fail("slot " + std::to_string(i) + " out of range");
     if (!b->be.set_modulus) return fail("this backend cannot replace a slot's modulus");
-    Mpz n, r, ninv, np, two;
+    Mpz n, np, two;
     from_signed_limbs(n.v, sign, limbs, nl);
     if (mpz_sgn(n.v) <= 0 || mpz_even_p(n.v) || mpz_cmp_ui(n.v, 1) <= 0) return fail("modulus must be odd and > 1");
     uint32_t pk, pl;
@@ -186,18 +221,13 @@ int mm_modbatch_set_modulus(mm_modbatch* b, size_t i, int sign, const uint32_t* 
     if (pl != b->len || pk > b->k)
         return fail("modulus needs transform length " + std::to_string(pl) + ", the batch has " + std::to_string(b->len));
     const uint32_t k = b->k;
-    mpz_setbit(r.v, 32 * (mp_bitcnt_t)k);
-    if (!mpz_invert(ninv.v, n.v, r.v)) return fail("modulus not invertible mod R");
-    mpz_sub(np.v, r.v, ninv.v);
-    mpz_mod(np.v, np.v, r.v);
+    MmModulus m;
+    mpz_set(m.n.v, n.v);
+    if (!montgomery_constants(n.v, k, np.v, m.r.v, m.rinv.v)) return fail("modulus not invertible mod R");
     mpz_mul_2exp(two.v, n.v, 1);
     std::vector<uint32_t> nl_(k), npl(k), tnl(k);
     if (!to_limbs(n.v, nl_.data(), k) || !to_limbs(np.v, npl.data(), k) || !to_limbs(two.v, tnl.data(), k))
         return fail("value does not fit " + std::to_string(k) + " limbs");
-    MmModulus m;
-    mpz_set(m.n.v, n.v);
-    mpz_mod(m.r.v, r.v, n.v);
-    if (!mpz_invert(m.rinv.v, r.v, n.v)) return fail("R not invertible mod N");
     if (b->be.set_modulus(b->state, i, nl_.data(), npl.data(), tnl.data())) return fail(b->be.last_error());
     mpz_set(b->moduli[i].n.v, m.n.v);
     mpz_set(b->moduli[i].r.v, m.r.v);
@@ -549,6 +579,53 @@ int mm_modbatch_check_pow(mm_modbatch* b, size_t i, uint32_t base, const uint32_
     mpz_set_ui(bz.v, base);
     mpz_powm(want.v, bz.v, ez.v, b->moduli[i].n.v);
     return mpz_cmp(got.v, want.v) == 0;
+}
+
+// check_pow on every slot: the raw limbs of all slots are read on the calling thread (the
+// backend is driven from one thread), then the reductions and GMP's powm run on `threads`
+// threads. Returns -1 when every slot has no bits left and reads base^e mod N, the first slot
+// that does not, or -2 on an error. exps: slot i's exponent is the little-endian limbs
+// exps[offsets[i] .. offsets[i+1]).
+int64_t mm_modbatch_check_pow_range(mm_modbatch* b, uint32_t base, size_t first_slot, size_t count,
+                                    const uint32_t* exps, const size_t* offsets, uint32_t threads) {
+    const size_t k = b->k;
+    if (first_slot > b->moduli.size() || count > b->moduli.size() - first_slot) {
+        fail("check_pow_range: slot range past the batch");
+        return -2;
+    }
+    std::vector<uint32_t> raw(count * k);
+    for (size_t i = 0; i < count; i++) {
+        if (b->be.remaining(b->state, first_slot + i) != 0) return (int64_t)(first_slot + i);
+        if (b->be.read(b->state, first_slot + i, raw.data() + i * k)) {
+            fail(b->be.last_error());
+            return -2;
+        }
+    }
+    const size_t slots = count;
+    std::atomic<size_t> next{0};
+    std::atomic<int64_t> first{-1};
+    auto work = [&] {
+        Mpz got, want, ez, bz;
+        mpz_set_ui(bz.v, base);
+        for (size_t i; (i = next.fetch_add(1)) < slots;) {
+            const MmModulus& m = b->moduli[first_slot + i];
+            mpz_import(got.v, k, -1, 4, 0, 0, raw.data() + i * k);
+            mpz_mul(got.v, got.v, m.rinv.v);
+            mpz_mod(got.v, got.v, m.n.v);
+            mpz_import(ez.v, offsets[i + 1] - offsets[i], -1, 4, 0, 0, exps + offsets[i]);
+            mpz_powm(want.v, bz.v, ez.v, m.n.v);
+            if (mpz_cmp(got.v, want.v) != 0) {
+                int64_t cur = first.load();
+                while ((cur < 0 || (int64_t)i < cur) && !first.compare_exchange_weak(cur, (int64_t)i)) {
+                }
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < std::max<uint32_t>(1, threads); t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    return first.load() < 0 ? -1 : (int64_t)first_slot + first.load();
 }
 
 // The score workload (score.rs:45-61): `slots` moduli of `bits` bits from GMP's default

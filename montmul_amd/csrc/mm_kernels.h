@@ -309,7 +309,9 @@ struct StoreLimbs {
 };
 
 // ---- Wraparound pieces --------------------------------------------------------------------
-// Position of spectral index (h - rev(p)) mod h, for bit-reversed positions p.
+// In a bit-reversed half spectrum of length h, the slot holding the conjugate-symmetric index
+// of the one at slot p: rev((h - rev(p)) mod h). It stays inside p's octave [2^t, 2^(t+1)) and
+// mirrors p there, which is 3 * 2^t - 1 - p, independent of h (lib.rs:530-541).
 __host__ __device__ __forceinline__ uint32_t partner(uint32_t p) {
     if (p == 0) return 0;
     uint32_t top = 31 - __builtin_clz(p);

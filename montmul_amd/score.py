@@ -1,6 +1,7 @@
-"""Squarings-per-second measurement of a backend on a fixed, seeded workload. A figure is returned
-only when sampled slots agree with pow_mod; repeats re-arm every slot, and the median and MAD of
-their rates are reported (reference: montmul/src/score.rs).
+"""The score: how many Montgomery squarings per second a backend sustains on a seeded batch of
+moduli. Every repeat re-arms all slots and times one run; the median rate and its median absolute
+deviation are the result, and none is given unless sampled slots equal 3^e mod N
+(reference: montmul/src/score.rs).
 """
 
 import random

@@ -8,6 +8,8 @@ methods); `ModBatch` and `fermat` are the host API under test.
 
 import random
 
+import pytest
+
 from montmul_amd import MixBatch, MontmulError, ModBatch, Params, fermat
 
 
@@ -195,6 +197,31 @@ def long_run(make, bits, slots, ebits, seed=5):
     for i, n in enumerate(moduli):
         assert batch.remaining(i) == 0, f"slot {i}"
         assert batch.read(i) == powmod(3, exps[i], n), f"bits {bits} slot {i}"
+
+
+def pow_range_check(make, bits, slots, ebits, seed=15):
+    """check_pow_range: -1 (None) when every slot of a finished run equals 3^e mod N, in one
+    part or several; the lowest wrong slot when one is changed; a slot with bits left fails."""
+    rng = random.Random(seed)
+    moduli = [rand_odd(rng, bits) for _ in range(slots)]
+    exps = [rand_exponent(rng, ebits) for _ in range(slots)]
+    batch = ModBatch(make(), moduli, 3)
+    for i, e in enumerate(exps):
+        batch.load(i, 1)
+        batch.set_exponent(i, e)
+    batch.run(ebits)
+    assert batch.check_pow_range(3, 0, exps, threads=3) is None
+    assert batch.check_pow_range(3, 2, exps[2:5]) is None
+    wrong = [slots - 2, slots - 1]
+    for i in wrong:
+        batch.load(i, powmod(3, exps[i], moduli[i]) ^ 2)
+    assert batch.check_pow_range(3, 0, exps, threads=4) == slots - 2
+    assert batch.check_pow_range(3, slots - 1, exps[-1:]) == slots - 1
+    batch.load(0, 1)
+    batch.set_exponent(0, 5)
+    assert batch.check_pow_range(3, 0, exps[:1]) == 0  # bits left
+    with pytest.raises(MontmulError, match="slot range past the batch"):
+        batch.check_pow_range(3, slots - 1, exps[:2])
 
 
 def end_to_end(make, ns):

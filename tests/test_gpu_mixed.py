@@ -121,3 +121,11 @@ def test_mixed_carry_out_lists_every_slot(hip_global):
     assert mix.failed_slots() == [1, 3]
     for i in (0, 2, 4):
         assert mix.read(i) == 4 % moduli[i]
+
+
+def test_mixed_1024_threads_with_radix4_global_length(hip_global):
+    """A 1,024-thread launch (L = 16,384 present) that also steps L = 32,768 slots: their batch
+    runs the register step on its own, but the radix-4 step in global mode inside the mixed
+    launch (whole-workgroup carries, a called function in the 1,024-thread build)."""
+    lengths = common.mixed_lengths(lambda: hip_global, [150000, 300000, 5000, 60000], 24, 9, seed=12)
+    assert lengths == [512, 4096, 16384, 32768]

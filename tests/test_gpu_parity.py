@@ -335,6 +335,23 @@ def test_headline_shape_closed_form(hip, hip_global):
     auto.close()
 
 
+def test_headline_all_4096_slots_against_powm(hip_global):
+    """The headline batch itself: 4,096 moduli of 100,000 bits from the score workload, the top
+    64 bits of N - 1 as exponents, one run(64) in global mode; every slot equals 3^e mod N
+    (GMP powm on 16 host threads, in parts of 1,024 slots)."""
+    moduli, exps = mm.score_workload(100_000, 4096, 64, 2026)
+    b = ModBatch(hip_global, moduli, 3)
+    for i, e in enumerate(exps):
+        b.load(i, 1)
+        b.set_exponent(i, e)
+    b.run(64)
+    for first in range(0, 4096, 1024):
+        say("headline slots", first)
+        assert b.check_pow_range(3, first, exps[first:first + 1024], threads=16) is None
+    common.pow_range_check(lambda: hip_global, 3000, 9, 60)
+    b.free()
+
+
 def test_cpu_case_4096_bits(hip):
     """The reference's CPU-runnable case: 64 moduli of 4,096 bits (L = 512, k = 256) from the
     score workload seeded 2026, e = (N - 1) >> 3,584; arm, run(16), re-arm, run(512). Every slot:

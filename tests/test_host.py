@@ -55,6 +55,11 @@ def test_fermat_tribonacci(name):
     common.end_to_end(BACKENDS[name], common.A092835_SMALL + [100, 200, 1001])
 
 
+@pytest.mark.parametrize("name", ["cpu_wrap", "gmp"])
+def test_check_pow_range(name):
+    common.pow_range_check(BACKENDS[name], 3000, 9, 60)
+
+
 def test_known_prps():
     common.known_prps(Gmp, common.A092835_SMALL, [8, 9, 10, 12, 13, 20, 86, 88, 803])
 
