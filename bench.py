@@ -32,13 +32,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "squarings/s at 100k-bit, 1 MI355X"
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-VALU_PEAK = 256 * 2.4e9    # wave64 VALU instructions/s: one per cycle per CU, 256 CUs at 2.4 GHz
-PROFILE = "profiles/k_steps_v2_global_100kbit_4096slots.md"
-# Per slot-step of the global-mode step kernel at L = 8,192, from PROFILE: HBM bytes moved
-# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, scales from a 1 GiB device copy in the same run) and
-# VALU wave instructions (SQ_INSTS_VALU).
-TRAFFIC_PER_SLOT_STEP = {8192: 506_735}
-VALU_PER_SLOT_STEP = {8192: 172_143}
+VALU_PEAK = 256 * 2.4e9    # wave64 VALU instructions/s at one per cycle per CU, 256 CUs at 2.4 GHz
+# Per slot-step of the global-mode step kernel, by transform length L, from the rocprofv3 summary
+# named in PROFILE: HBM bytes moved (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, scales from a
+# 1 GiB device copy in the same run) and VALU wave instructions (SQ_INSTS_VALU). Counters cannot
+# be read from inside this process; the summaries are re-measured whenever the kernel changes.
+PROFILE = {8192: "profiles/k_steps_v2_global_100kbit_4096slots.md",
+           32768: "profiles/k_steps_v2_split_300kbit_1024slots.md"}
+TRAFFIC_PER_SLOT_STEP = {8192: 385_809, 32768: 9_757_633}
+VALU_PER_SLOT_STEP = {8192: 170_483, 32768: 832_892}
 REF_5090 = 1_536_441.0     # reference, RTX 5090, 170 slots (README.md:14)
 
 
@@ -261,12 +263,13 @@ def main():
         "ms_per_step": round(1e3 * statistics.mean(head["secs"]), 3),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(value / REF_5090, 4),
+        "vs_baseline": round(value / REF_5090, 4) if args.bits == 100_000 else None,
         "vs_baseline_ref": "reference Gpu backend, RTX 5090, 170 slots, 1,536,441 squarings/s (README.md:14)",
         "dtype": "u32",
         "data": "synthetic (GMP generator seeded 2026, as the reference's score workload)",
-        "config": {"workload": "100,000-bit moduli, L=8192, k=4096, base 3, top 2,000 bits of N-1, "
-                               "global mode (T, x and constants in HBM; transform exchanges through LDS tiles of at most 2,048 elements)",
+        "config": {"workload": f"{args.bits:,}-bit moduli, L={L}, k={head['k']}, base 3, top {ebits:,} bits of N-1, " +
+                               ("global mode (T, x and constants in HBM; transform exchanges through LDS tiles)"
+                                if "global" in results else f"{modes[0]} mode"),
                    "bits": args.bits, "L": L, "k": head["k"], "slots": slots, "ebits": ebits,
                    "mode": "global" if "global" in results else modes[0],
                    "threads_per_wg": head["plan"]["threads"], "lds_bytes": head["plan"]["lds_bytes"]},
@@ -282,7 +285,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5),
                      "traffic": (TRAFFIC_PER_SLOT_STEP[L] * slots * ebits) if L in TRAFFIC_PER_SLOT_STEP else None,
-                     "traffic_per_slot_step": TRAFFIC_PER_SLOT_STEP.get(L), "traffic_source": PROFILE,
+                     "traffic_per_slot_step": TRAFFIC_PER_SLOT_STEP.get(L), "traffic_source": PROFILE.get(L),
                      "kernel": "k_steps_v2 (global mode)", "kernel_ms_per_launch": round(kms, 3),
                      "alg_bytes_per_slot_step": 28 * L},
         "checked_slots": head["checked"],
@@ -292,7 +295,7 @@ def main():
         "valu": ({"bound": "valu", "achieved": round(VALU_PER_SLOT_STEP[L] * slots * ebits / (kms * 1e-3) / 1e9, 2),
                   "peak": VALU_PEAK / 1e9, "unit": "G wave-instr/s",
                   "frac": round(VALU_PER_SLOT_STEP[L] * slots * ebits / (kms * 1e-3) / VALU_PEAK, 4),
-                  "instr_per_slot_step": VALU_PER_SLOT_STEP[L], "source": PROFILE}
+                  "instr_per_slot_step": VALU_PER_SLOT_STEP[L], "source": PROFILE[L]}
                  if L in VALU_PER_SLOT_STEP else None),
         "sweep": head["sweep"],
     }

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void k_probe_issue(uint32_t* out, uint32_t ite
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t a = (t ^ seed) | 1u, b = a * 2654435761u + seed;
     uint64_t c[8];
-    uint32_t d[8];  // the independent 32-bit chains of OP_MAD_AND_ADD
+    uint32_t d[8];  // the 32-bit chains (OP_ADD_U32, OP_MUL_HI_U32, OP_MAD_AND_ADD)
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         c[j] = (uint64_t)(b + 77u * (uint32_t)j) << 7 | (uint32_t)j;
@@ -42,13 +42,9 @@ __global__ __launch_bounds__(256) void k_probe_issue(uint32_t* out, uint32_t ite
                 uint64_t carry;  // the instruction's carry-out pair, unused
                 asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(c[j]), "=s"(carry) : "v"(a), "v"(b));
             } else if constexpr (OP == OP_ADD_U32) {
-                uint32_t x = (uint32_t)c[j];
-                asm volatile("v_add_u32 %0, %0, %1" : "+v"(x) : "v"(a));
-                c[j] = x;
+                asm volatile("v_add_u32 %0, %0, %1" : "+v"(d[j]) : "v"(a));
             } else if constexpr (OP == OP_MUL_HI_U32) {
-                uint32_t x = (uint32_t)c[j];
-                asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(x) : "v"(a));
-                c[j] = x;
+                asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(d[j]) : "v"(a));
             } else if constexpr (OP == OP_ADD_U64) {
                 const uint64_t k = (uint64_t)a << 3 | (uint32_t)j;
                 asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(c[j]) : "v"(k));
@@ -63,7 +59,7 @@ __global__ __launch_bounds__(256) void k_probe_issue(uint32_t* out, uint32_t ite
     }
     uint64_t r = 0;
 #pragma unroll
-    for (int j = 0; j < 8; j++) r ^= c[j] + (OP == OP_MAD_AND_ADD ? d[j] : 0u);
+    for (int j = 0; j < 8; j++) r ^= c[j] + d[j];
     out[t] = (uint32_t)r ^ (uint32_t)(r >> 32);
 }
 

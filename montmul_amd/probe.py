@@ -3,7 +3,7 @@
     python -m montmul_amd.probe [--iters 20000] [--blocks 0] [--ops 0,1,2,3,4]
 
 Each op runs, per thread, 8 independent chains of one instruction for `iters` iterations, over
-`blocks` workgroups of 256 threads (0: CUs x 32, i.e. 8 waves per SIMD). Prints one JSON line
+`blocks` workgroups of 256 threads (0: CUs x 32, which is 128 waves per CU over the launch). Prints one JSON line
 per op: device time, chained instructions per second (waves x 8 x iters / time) and that rate
 per CU per cycle at the clock given by --clock-ghz (the counter run under rocprofv3 gives the
 measured clock and SQ_INSTS_VALU). The step kernel's roofline divides its own instruction rate
