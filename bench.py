@@ -32,7 +32,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "squarings/s at 100k-bit, 1 MI355X"
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-VALU_PEAK = 256 * 2.4e9    # wave64 VALU instructions/s at one per cycle per CU, 256 CUs at 2.4 GHz
+# Attainable VALU issue of the step kernel's own instruction mix, in wave instructions per CU
+# per cycle: the issue rates measured by mm_probe_issue (32-bit operations 1.37, 64-bit 0.95)
+# weighted by the kernel's INT32 / INT64 / other counts (profiles/valu_issue_probe.md), over
+# 256 CUs at 2.4 GHz.
+VALU_MIX_PER_CU_CYCLE = {8192: 1.22}
+VALU_PROBE = "profiles/valu_issue_probe.md"
 # Per slot-step of the global-mode step kernel, by transform length L, from the rocprofv3 summary
 # named in PROFILE: HBM bytes moved (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, scales from a
 # 1 GiB device copy in the same run) and VALU wave instructions (SQ_INSTS_VALU). Counters cannot
@@ -145,6 +150,19 @@ def cpu_baseline(bits, threads, gmp_steps=8_000, ntt_steps=1_000, single_steps=2
     r1, dt1 = rate(Gmp(threads=1), 1, single_steps)
     out["gmp_single"] = (r1, 1, dt1)
     out["steps"] = {"gmp": gmp_steps, "cpu_ntt": ntt_steps, "gmp_single": single_steps}
+    return out
+
+
+def valu_roof(L, slot_steps_per_s):
+    """The kernel's VALU issue rate (instructions per slot-step from the profile, rate from this
+    run's kernel time) against the rate its instruction mix can issue, where measured."""
+    achieved = VALU_PER_SLOT_STEP[L] * slot_steps_per_s
+    out = {"bound": "valu", "achieved": round(achieved / 1e9, 2), "unit": "G wave-instr/s",
+           "instr_per_slot_step": VALU_PER_SLOT_STEP[L], "source": PROFILE[L],
+           "per_cu_per_cycle_at_2.4GHz": round(achieved / (256 * 2.4e9), 4)}
+    if L in VALU_MIX_PER_CU_CYCLE:
+        peak = VALU_MIX_PER_CU_CYCLE[L] * 256 * 2.4e9
+        out.update(peak=round(peak / 1e9, 1), frac=round(achieved / peak, 4), peak_source=VALU_PROBE)
     return out
 
 
@@ -292,11 +310,7 @@ def main():
         "checked_all_slots": head.get("all_slots"),
         # The bound that binds: VALU issue (integer work, no MFMA). Instructions per slot-step
         # from the profile, rate from this run's kernel time.
-        "valu": ({"bound": "valu", "achieved": round(VALU_PER_SLOT_STEP[L] * slots * ebits / (kms * 1e-3) / 1e9, 2),
-                  "peak": VALU_PEAK / 1e9, "unit": "G wave-instr/s",
-                  "frac": round(VALU_PER_SLOT_STEP[L] * slots * ebits / (kms * 1e-3) / VALU_PEAK, 4),
-                  "instr_per_slot_step": VALU_PER_SLOT_STEP[L], "source": PROFILE[L]}
-                 if L in VALU_PER_SLOT_STEP else None),
+        "valu": (valu_roof(L, slots * ebits / (kms * 1e-3)) if L in VALU_PER_SLOT_STEP else None),
         "sweep": head["sweep"],
     }
     if "lds" in results and "global" in results:

@@ -80,10 +80,10 @@ struct WholeBlock {
     static __device__ __forceinline__ uint32_t t() { return threadIdx.x; }
     static __device__ __forceinline__ uint32_t n() { return blockDim.x; }
 };
-// The lane id (mbcnt over a full exec mask) equals threadIdx.x & 63 for whole-wave groups. In
-// the 1,024-thread mixed-length kernel, whose register pressure spills the work-item ID, the
-// step computed from threadIdx.x & 63 came out wrong (every limb, deterministically) while
-// the lane id gave the right results; the 512-thread build was right either way.
+// The lane id (mbcnt over a full exec mask) equals threadIdx.x & 63 for whole-wave groups and
+// needs no work-item ID. An earlier, heavily spilling build of the 1,024-thread mixed-length
+// kernel stepped wrongly with threadIdx.x & 63 here and exactly with the lane id; builds of the
+// present source are exact with either.
 struct OneWave {
     static constexpr bool grouped = true;
     static __device__ __forceinline__ uint32_t t() { return __lane_id(); }
