@@ -113,8 +113,14 @@ typedef struct mm_options {
                               * LDS footprint fits more workgroups per CU and is faster there */
     int32_t autotune;        /* time every plan that can run a new batch's length (radix-8
                               * register kernel or radix-4 kernel, global mode with LDS tiles or
-                              * LDS-resident) for 16 steps on the batch's own slots and keep the
-                              * fastest; the choice is cached per length and slot-count octave */
+                              * LDS-resident, the register kernel's exchange tile, one or as many
+                              * slots per CU as fit) for 16 steps on the batch's own slots and keep
+                              * the fastest; the choice is cached per length and slot-count octave */
+    uint32_t tile;           /* without autotune: the register kernel's exchange tile in global
+                              * mode, in elements (0: the default, 2,048 up to L = 8,192 and 4,096
+                              * above; 4,096 is also built for L = 8,192) */
+    uint32_t slots_per_cu;   /* without autotune: 1 pads each workgroup's LDS so that one slot runs
+                              * per CU; 0: as many as fit */
 } mm_options;
 
 /* Launch plan of a batch. Replaces `LaunchPlan`/`Mode` (gpu/plan.rs:5-22). */
@@ -125,6 +131,8 @@ typedef struct mm_plan_info {
     uint32_t slots_per_wg;   /* slots handled by one workgroup */
     uint32_t kernel;         /* 4: radix-4 step (LDS or global); 8: register-resident radix-8 step */
     int32_t tuned;           /* 1 when the autotuner chose this plan */
+    uint32_t tile;           /* register kernel in global mode: exchange tile in elements; else 0 */
+    uint32_t slots_per_cu;   /* 0: as many workgroups per CU as fit; 1: LDS padded to one per CU */
 } mm_plan_info;
 
 /* Replaces Gpu::device_count (gpu/mod.rs:70-72). Returns -1 on error. */
@@ -168,9 +176,10 @@ int64_t mm_run_group(void *const *states, size_t count, uint64_t bits, size_t *w
 /* Device time of the step kernel launches of the latest run, in milliseconds (HIP events on
  * the batch's stream), and the number of launches. No reference counterpart: measurement. */
 int mm_last_run_kernel_ms(void *state, double *ms, uint32_t *launches);
-/* The squaring's output of slot `slot`'s latest step: T = c x^2 as 2k limbs (c = base on a 1 bit,
- * else 1), which global mode keeps in HBM; after mm_run_bare, x is the step's input. Fails in
- * LDS-resident mode. No reference counterpart: it lets tests check the squaring half alone. */
+/* After mm_run_bare in global mode, the squaring's output of slot `slot`'s latest step: T = c x^2
+ * as 2k limbs (c = base on a 1 bit, else 1); x is the step's input. Fails in LDS-resident mode
+ * and after a full run (whose steps keep T on chip where they can). No reference counterpart:
+ * it lets tests check the squaring half alone. */
 int mm_read_product(void *state, size_t slot, uint32_t *out);
 /* The Hip table's failed_slots (see mm_backend). */
 size_t mm_failed_slots(void *state, uint32_t *out, size_t max);

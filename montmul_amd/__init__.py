@@ -81,7 +81,8 @@ _CLOSE = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(BackendTable))
 
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("force_global", C.c_int32), ("wraparound", C.c_int32),
-                ("max_launch_ms", C.c_uint32), ("prefer_lds", C.c_int32), ("autotune", C.c_int32)]
+                ("max_launch_ms", C.c_uint32), ("prefer_lds", C.c_int32), ("autotune", C.c_int32),
+                ("tile", C.c_uint32), ("slots_per_cu", C.c_uint32)]
 
 
 class ServerOptions(C.Structure):
@@ -105,7 +106,8 @@ class ServerStats(C.Structure):
 
 class PlanInfo(C.Structure):
     _fields_ = [("global_", C.c_int32), ("threads", C.c_uint32), ("lds_bytes", C.c_uint32),
-                ("slots_per_wg", C.c_uint32), ("kernel", C.c_uint32), ("tuned", C.c_int32)]
+                ("slots_per_wg", C.c_uint32), ("kernel", C.c_uint32), ("tuned", C.c_int32),
+                ("tile", C.c_uint32), ("slots_per_cu", C.c_uint32)]
 
 
 def _load_lib():
@@ -285,11 +287,13 @@ class Hip(Backend):
     """The MI355X backend (replaces `Gpu`/`GpuOptions`, gpu/mod.rs:28-111)."""
 
     def __init__(self, device=0, force_global=False, wraparound=True, max_launch_ms=0, prefer_lds=False,
-                 autotune=False):
+                 autotune=False, tile=0, slots_per_cu=0):
         """autotune: time every plan that fits each new batch's length (16 steps on its own
-        slots) and keep the fastest, cached per length and slot-count octave."""
+        slots) and keep the fastest, cached per length and slot-count octave. Without it, `tile`
+        (the register kernel's exchange tile in global mode, in elements; 0 = default) and
+        `slots_per_cu` (1 = one slot per CU; 0 = as many as fit) fix those choices."""
         self.options = Options(device, int(bool(force_global)), int(bool(wraparound)), max_launch_ms,
-                               int(bool(prefer_lds)), int(bool(autotune)))
+                               int(bool(prefer_lds)), int(bool(autotune)), tile, slots_per_cu)
         self.table = BackendTable()
         if lib.mm_hip_open(C.byref(self.options), C.byref(self.table)) != 0:
             raise MontmulError((lib.mm_last_error() or b"").decode())
@@ -333,7 +337,8 @@ class Hip(Backend):
         p = PlanInfo()
         lib.mm_plan(batch.state, C.byref(p))
         return {"global": bool(p.global_), "threads": p.threads, "lds_bytes": p.lds_bytes,
-                "slots_per_wg": p.slots_per_wg, "kernel": p.kernel, "tuned": bool(p.tuned)}
+                "slots_per_wg": p.slots_per_wg, "kernel": p.kernel, "tuned": bool(p.tuned),
+                "tile": p.tile, "slots_per_cu": p.slots_per_cu}
 
     @staticmethod
     def run_bare(batch, bits):

@@ -45,10 +45,10 @@ __global__ __launch_bounds__(1024) void k_steps(KArgs A, uint32_t first, uint32_
 
 // The kernels are held to 128 VGPRs (four waves per SIMD), so two 512-thread workgroups share a
 // CU wherever LDS allows.
-template <uint32_t LOGN, bool GLOBAL, bool BARE>
+template <uint32_t LOGN, bool GLOBAL, bool BARE, uint32_t TILE = 1>
 __global__ __launch_bounds__(LOGN > 12 ? 1024 : 512) __attribute__((amdgpu_waves_per_eu(MM_V2_WAVES_PER_EU))) void k_steps_v2(v2::VArgs A, uint32_t first, uint32_t bits) {
     extern __shared__ __align__(16) unsigned char lds[];
-    steps_v2<LOGN, GLOBAL, BARE>(A, first + blockIdx.x, bits, lds);
+    steps_v2<LOGN, GLOBAL, BARE, false, TILE, true>(A, first + blockIdx.x, bits, lds);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t count, uint32_t v) {
@@ -459,6 +459,14 @@ struct DevTables {
     }
 };
 
+// A launch plan the autotuner can choose: kernel (register radix-8 or radix-4), mode (global or
+// LDS-resident), the register kernel's exchange tile in global mode (1: its default) and slots
+// per CU (0: as many as fit; 1: one, by padding the workgroup's LDS past half a CU's).
+struct PlanChoice {
+    bool v2 = false, global = false;
+    uint32_t tile = 1, slots_per_cu = 0;
+};
+
 struct Ctx {
     mm_options opts{};
     int device = 0;
@@ -474,7 +482,7 @@ struct Ctx {
     size_t mix_cap = 0;
     double mix_ms_per_step = 0;
     // Autotuner choices: (n, wraparound, slot-count octave) -> (register kernel, global mode).
-    std::map<std::tuple<uint32_t, bool, uint32_t>, std::pair<bool, bool>> tuned;
+    std::map<std::tuple<uint32_t, bool, uint32_t>, PlanChoice> tuned;
     std::string name;
     std::mutex mu;
     std::map<uint32_t, std::shared_ptr<DevTables>> tabs;
@@ -572,7 +580,7 @@ struct State {
     int device = 0;  // C's device, kept so that a state can be freed after its context is closed
     uint32_t k = 0, len = 0, n = 0, base = 0, slots = 0;
     bool global = false, wrap = true, v2 = false;
-    uint32_t threads = 0;
+    uint32_t threads = 0, tile = 1, slots_per_cu = 0;
     size_t lds = 0;
     std::shared_ptr<DevTables> tab, half;
     // device buffers
@@ -604,6 +612,7 @@ struct State {
     bool tuned = false;
     double last_ms = 0;
     uint32_t last_launches = 0;
+    bool product_in_hbm = false;  // the latest run was run_bare: d_lim holds its last product
     std::vector<uint32_t> failed;     // every slot whose carry check failed in the latest run or mul
     ~State() {
         hipSetDevice(device);
@@ -656,7 +665,16 @@ template <uint32_t LOGN> StepKernelV2 pick_v2(bool global, bool bare) {
     if (global) return bare ? k_steps_v2<LOGN, true, true> : k_steps_v2<LOGN, true, false>;
     return bare ? k_steps_v2<LOGN, false, true> : k_steps_v2<LOGN, false, false>;
 }
-StepKernelV2 pick_step_v2(uint32_t logn, bool global, bool bare) {
+// The exchange tiles the autotuner may pick besides the default (v2::tile_max): one tile of the
+// whole vector at n = 4,096 in global mode (one chunk per exchange, one workgroup per CU).
+constexpr uint32_t V2_WIDE_TILE = 4096;
+bool v2_tile_ok(uint32_t n, bool global, uint32_t tile) { return tile == 1 || (global && tile == V2_WIDE_TILE && n == 4096); }
+StepKernelV2 pick_step_v2(uint32_t logn, bool global, bool bare, uint32_t tile = 1) {
+    if (tile != 1) {
+        if (logn == 12 && global && tile == V2_WIDE_TILE)
+            return bare ? k_steps_v2<12, true, true, V2_WIDE_TILE> : k_steps_v2<12, true, false, V2_WIDE_TILE>;
+        return nullptr;
+    }
     switch (logn) {
         case 9: return pick_v2<9>(global, bare);
         case 10: return pick_v2<10>(global, bare);
@@ -694,8 +712,8 @@ int set_lds_limit(Ctx* C) {
         }
         dyn_max = std::min(dyn_max, dyn);
     }
-    for (uint32_t i = 0; i < 24; i++) {
-        StepKernelV2 k = pick_step_v2(9 + i / 4, (i >> 1) & 1, i & 1);
+    for (uint32_t i = 0; i < 26; i++) {
+        StepKernelV2 k = i < 24 ? pick_step_v2(9 + i / 4, (i >> 1) & 1, i & 1) : pick_step_v2(12, true, i & 1, V2_WIDE_TILE);
         if (!k) continue;
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, (const void*)k));
@@ -880,13 +898,16 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
     S->failed.clear();
     if (left == 0) return MM_OK;
     if (push(S)) return MM_EFAULT;
+    // Only run_bare leaves each step's product in HBM: a full step of the register kernel keeps
+    // it on chip where it can.
+    S->product_in_hbm = bare;
     KArgs A = S->args();
     StepKernel kern = pick_step(S->global, S->wrap, bare);
     v2::VArgs A2{};
     StepKernelV2 kern2 = nullptr;
     if (S->v2) {
         A2 = S->vargs();
-        kern2 = pick_step_v2(ilog2(S->n), S->global, bare);
+        kern2 = pick_step_v2(ilog2(S->n), S->global, bare, S->tile);
     }
     while (left > 0) {
         // Steps per launch: all of them, or about max_launch_ms worth from the latest rate.
@@ -924,49 +945,82 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
 }
 
 // ---- Plans and the autotuner ------------------------------------------------------------------
+// Dynamic LDS at which a second workgroup no longer fits a CU beside the first.
+uint64_t one_per_cu_lds(const Ctx* C) { return C->max_lds / 2 + 2048; }
+
 // The launch shape of a plan: the register kernel (n/8 threads; an exchange tile in global mode,
 // the whole vector and T resident otherwise) or the radix-4 kernel (one radix-4 group per
-// thread; the vector resident in LDS, or only the carry scratch in global mode).
-void set_plan(State* S, bool v2, bool global) {
+// thread; the vector resident in LDS, or only the carry scratch in global mode). With one slot
+// per CU the workgroup's dynamic LDS is padded past half of a CU's, so no second one fits.
+void set_plan(State* S, const PlanChoice& p) {
     const uint32_t n = S->n, k = S->k;
-    S->v2 = v2;
-    S->global = global;
-    if (v2) {
+    S->v2 = p.v2;
+    S->global = p.global;
+    S->tile = p.v2 && p.global ? p.tile : 1;
+    S->slots_per_cu = p.slots_per_cu;
+    if (p.v2) {
         S->threads = std::min<uint32_t>(n / 8, 1024);  // n = 2^14: two halves of n/2, n/16 threads
-        const uint64_t tile = std::min<uint32_t>(n, n > 4096 ? 4096 : 2048);  // v2::Tiling
-        S->lds = global ? 24 * tile + lds_scratch_bytes(S->threads) : v2::v2_lds_resident(n, k, S->threads);
+        const uint64_t want = S->tile != 1 ? S->tile : (n > 4096 ? 4096 : 2048);  // v2::Tiling
+        const uint64_t tile = std::min<uint64_t>(n, want);
+        S->lds = p.global ? 24 * tile + lds_scratch_bytes(S->threads) : v2::v2_lds_resident(n, k, S->threads);
     } else {
         S->threads = plan_threads(n);
-        S->lds = global ? lds_scratch_bytes(S->threads) : lds_bytes_resident(n, k, S->threads);
+        S->lds = p.global ? lds_scratch_bytes(S->threads) : lds_bytes_resident(n, k, S->threads);
     }
+    if (p.slots_per_cu == 1) S->lds = std::max<size_t>(S->lds, one_per_cu_lds(S->C));
+}
+void set_plan(State* S, bool v2, bool global) {
+    PlanChoice p;
+    p.v2 = v2;
+    p.global = global;
+    set_plan(S, p);
 }
 
 // Whether a plan can run the batch: the register kernel needs the wraparound path and
-// n = 512 .. 8,192; an LDS-resident plan needs its bytes within the kernel's dynamic LDS.
-bool plan_fits(const State* S, bool v2, bool global) {
+// n = 512 .. 16,384 (16,384 in global mode only); an LDS-resident plan, or a padded one, needs
+// its bytes within the kernel's dynamic LDS; a wider tile is built for n = 4,096 only.
+bool plan_fits(const State* S, const PlanChoice& p) {
     const Ctx* C = S->C;
-    if (v2 && !(S->wrap && S->n >= 512 && S->n <= 16384)) return false;
-    if (v2 && S->n > 8192 && !global) return false;  // n = 2^14: global mode only
-    if (global) return true;
-    if (v2) return v2::v2_lds_resident(S->n, S->k, S->n / 8) <= C->max_dyn_lds_v2;
-    return lds_bytes_resident(S->n, S->k, plan_threads(S->n)) <= C->max_dyn_lds;
+    if (p.v2 && !(S->wrap && S->n >= 512 && S->n <= 16384)) return false;
+    if (p.v2 && S->n > 8192 && !p.global) return false;  // n = 2^14: global mode only
+    if (p.v2 && !v2_tile_ok(S->n, p.global, p.tile)) return false;
+    if (!p.v2 && p.tile != 1) return false;
+    uint64_t lds;
+    if (p.v2) {
+        const uint64_t want = p.tile != 1 ? p.tile : (S->n > 4096 ? 4096 : 2048);
+        lds = p.global ? 24 * std::min<uint64_t>(S->n, want) + lds_scratch_bytes(std::min<uint32_t>(S->n / 8, 1024))
+                       : v2::v2_lds_resident(S->n, S->k, S->n / 8);
+    } else {
+        lds = p.global ? lds_scratch_bytes(plan_threads(S->n)) : lds_bytes_resident(S->n, S->k, plan_threads(S->n));
+    }
+    if (p.slots_per_cu == 1) lds = std::max<uint64_t>(lds, one_per_cu_lds(C));
+    return lds <= (p.v2 ? C->max_dyn_lds_v2 : C->max_dyn_lds);
+}
+bool plan_fits(const State* S, bool v2, bool global) {
+    PlanChoice p;
+    p.v2 = v2;
+    p.global = global;
+    return plan_fits(S, p);
 }
 
-// Times every plan that fits for 16 steps on the batch's own slots and keeps the fastest. The
-// slots hold x = 0 and a 16-bit zero exponent for the timing (every kernel's work is the same
-// for any data, and 0 stays 0), and are cleared afterwards. The choice is cached per (n,
-// wraparound, slot-count octave), so later batches of that shape take it without timing.
+// Times every plan that fits for 16 steps on the batch's own slots and keeps the fastest:
+// kernel × mode × (register kernel in global mode) exchange tile × slots per CU. The slots hold
+// x = 0 and a 16-bit zero exponent for the timing (every kernel's work is the same for any data,
+// and 0 stays 0), and are cleared afterwards. The choice is cached per (n, wraparound,
+// slot-count octave), so later batches of that shape take it without timing.
 int autotune(State* S, bool v2ok) {
     Ctx* C = S->C;
     const auto key = std::make_tuple(S->n, S->wrap, ilog2(S->slots));
-    bool have = false, best_v2 = S->v2, best_global = S->global;
+    bool have = false;
+    PlanChoice best;
+    best.v2 = S->v2;
+    best.global = S->global;
     {
         std::lock_guard<std::mutex> g(C->mu);
         auto it = C->tuned.find(key);
         if (it != C->tuned.end()) {
             have = true;
-            best_v2 = it->second.first;
-            best_global = it->second.second;
+            best = it->second;
         }
     }
     if (!have) {
@@ -974,42 +1028,51 @@ int autotune(State* S, bool v2ok) {
         hipStream_t sm = C->stream;
         std::vector<uint32_t> nb(S->slots, T);
         HIPCHK(hipMemcpyAsync(S->d_nbits, nb.data(), S->slots * 4ull, hipMemcpyHostToDevice, sm));
-        double best = 1e300;
+        double best_ms = 1e300;
+        std::vector<PlanChoice> cands;
         for (int v = v2ok ? 1 : 0; v >= 0; v--)
-            for (int g = 1; g >= 0; g--) {
-                if (!plan_fits(S, v, g)) continue;
-                set_plan(S, v, g);
-                double t = 1e300;
-                for (int rep = 0; rep < 2; rep++) {  // the first launch also loads the code
-                    HIPCHK(hipMemsetAsync(S->d_pos, 0, S->slots * 4ull, sm));
-                    HIPCHK(hipEventRecord(C->ev0, sm));
-                    if (v)
-                        hipLaunchKernelGGL(pick_step_v2(ilog2(S->n), g, false), dim3(S->slots), dim3(S->threads), S->lds, sm,
-                                           S->vargs(), 0u, T);
-                    else
-                        hipLaunchKernelGGL(pick_step(g, S->wrap, false), dim3(S->slots), dim3(S->threads), S->lds, sm,
-                                           S->args(), 0u, T);
-                    HIPCHK(hipGetLastError());
-                    HIPCHK(hipEventRecord(C->ev1, sm));
-                    HIPCHK(hipEventSynchronize(C->ev1));
-                    float ms = 0;
-                    hipEventElapsedTime(&ms, C->ev0, C->ev1);
-                    t = std::min(t, (double)ms);
-                }
-                if (t < best) {
-                    best = t;
-                    best_v2 = v;
-                    best_global = g;
-                }
+            for (int g = 1; g >= 0; g--)
+                for (uint32_t tile : {1u, V2_WIDE_TILE})
+                    for (uint32_t spc : {0u, 1u}) {
+                        PlanChoice p;
+                        p.v2 = v;
+                        p.global = g;
+                        p.tile = tile;
+                        p.slots_per_cu = spc;
+                        if (plan_fits(S, p)) cands.push_back(p);
+                    }
+        for (const PlanChoice& p : cands) {
+            set_plan(S, p);
+            double t = 1e300;
+            for (int rep = 0; rep < 2; rep++) {  // the first launch also loads the code
+                HIPCHK(hipMemsetAsync(S->d_pos, 0, S->slots * 4ull, sm));
+                HIPCHK(hipEventRecord(C->ev0, sm));
+                if (p.v2)
+                    hipLaunchKernelGGL(pick_step_v2(ilog2(S->n), p.global, false, S->tile), dim3(S->slots), dim3(S->threads),
+                                       S->lds, sm, S->vargs(), 0u, T);
+                else
+                    hipLaunchKernelGGL(pick_step(p.global, S->wrap, false), dim3(S->slots), dim3(S->threads), S->lds, sm,
+                                       S->args(), 0u, T);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventRecord(C->ev1, sm));
+                HIPCHK(hipEventSynchronize(C->ev1));
+                float ms = 0;
+                hipEventElapsedTime(&ms, C->ev0, C->ev1);
+                t = std::min(t, (double)ms);
             }
+            if (t < best_ms) {
+                best_ms = t;
+                best = p;
+            }
+        }
         HIPCHK(hipMemsetAsync(S->d_nbits, 0, S->slots * 4ull, sm));
         HIPCHK(hipMemsetAsync(S->d_pos, 0, S->slots * 4ull, sm));
         HIPCHK(hipMemsetAsync(S->d_err, 0, S->slots * 4ull, sm));
         HIPCHK(hipStreamSynchronize(sm));
         std::lock_guard<std::mutex> g(C->mu);
-        C->tuned[key] = {best_v2, best_global};
+        C->tuned[key] = best;
     }
-    set_plan(S, best_v2, best_global);
+    set_plan(S, best);
     S->tuned = true;
     return 0;
 }
@@ -1294,6 +1357,17 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         set_plan(S.get(), false, C->opts.force_global || !plan_fits(S.get(), false, false));
     }
     const bool tune = C->opts.autotune && !C->opts.force_global;
+    if (!tune && (C->opts.tile != 0 || C->opts.slots_per_cu != 0)) {
+        PlanChoice p;
+        p.v2 = S->v2;
+        p.global = S->global;
+        p.tile = C->opts.tile != 0 ? C->opts.tile : 1;
+        p.slots_per_cu = C->opts.slots_per_cu;
+        if (p.slots_per_cu > 1 || !plan_fits(S.get(), p))
+            return fail("tile " + std::to_string(C->opts.tile) + " / slots per CU " + std::to_string(C->opts.slots_per_cu) +
+                        " is not a plan for L = " + std::to_string(st->len));
+        set_plan(S.get(), p);
+    }
     // Device memory: x, N'-hat, N-hat or A|B, work vectors and limbs in global mode, exponents
     // of up to k words, per-slot words, tables (about 6n elements per field, twice), a margin.
     const uint64_t per_slot = 4ull * k + 48ull * n + (S->global || tune ? 24ull * n + 12ull * k : 0) + 16 + 4ull * k +
@@ -1563,6 +1637,8 @@ int mm_plan(void* st, mm_plan_info* out) {
     out->slots_per_wg = 1;
     out->kernel = S->v2 ? 8 : 4;
     out->tuned = S->tuned ? 1 : 0;
+    out->tile = S->v2 && S->global ? (S->tile != 1 ? S->tile : std::min<uint32_t>(S->n, S->n > 4096 ? 4096 : 2048)) : 0;
+    out->slots_per_cu = S->slots_per_cu;
     return 0;
 }
 
@@ -1585,6 +1661,7 @@ int mm_read_product(void* st, size_t i, uint32_t* out) {
     State* S = (State*)st;
     if (i >= S->slots) return fail("slot out of range");
     if (!S->global || !S->d_lim) return fail("the product stays in HBM only in global mode");
+    if (!S->product_in_hbm) return fail("the product is left in HBM by run_bare only");
     if (hipSetDevice(S->C->device) != hipSuccess) return fail("hipSetDevice failed");
     HIPCHK(hipMemcpyAsync(out, S->d_lim + i * 3 * (size_t)S->k, 2ull * S->k * 4, hipMemcpyDeviceToHost, S->C->stream));
     HIPCHK(hipStreamSynchronize(S->C->stream));

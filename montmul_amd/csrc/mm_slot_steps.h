@@ -64,8 +64,12 @@ __device__ __forceinline__ void steps_v1(const KArgs& A, uint32_t slot, uint32_t
 // the exchanges use an LDS tile of at most 2,048 elements (4,096 at n = 8,192 and 16,384, where
 // the vector is two sub-transforms of n/2 in turn). WIDE: the workgroup has more than n / 8
 // threads (see v2_square).
-template <uint32_t LOGN, bool GLOBAL, bool BARE, bool WIDE = false>
+// TILE: the exchange tile in global mode (1: the default of v2::tile_max; else its size in
+// elements, as the autotuner picks it). REG_T: T may stay in registers from the square to the
+// reduction (the step kernel of one length; the mixed-length kernel has no registers to spare).
+template <uint32_t LOGN, bool GLOBAL, bool BARE, bool WIDE = false, uint32_t TILE = 1, bool REG_T = false>
 __device__ __forceinline__ void steps_v2(const v2::VArgs& A, uint32_t slot, uint32_t bits, unsigned char* lds) {
+    constexpr uint32_t TILED = GLOBAL ? TILE : 0;
     const Slots& S = A.s;
     const uint32_t n = A.t.P.n, k = S.k;
     v2::Xbuf X;
@@ -74,7 +78,7 @@ __device__ __forceinline__ void steps_v2(const v2::VArgs& A, uint32_t slot, uint
     if (GLOBAL) {
         // Transform vectors in registers between exchanges, exchanges through an LDS tile,
         // T and x in HBM.
-        constexpr uint32_t tile = v2::Tiling<LOGN, true>::tile;
+        constexpr uint32_t tile = v2::Tiling<LOGN, TILED>::tile;
         X.z2 = (v2::Z2*)lds;
         X.z1 = (v2::Z1*)(lds + 16 * tile);
         Tl = S.lim + (size_t)slot * 3 * k;
@@ -108,10 +112,15 @@ __device__ __forceinline__ void steps_v2(const v2::VArgs& A, uint32_t slot, uint
                                                    S.nphat2 + (size_t)slot * n, S.nhat1 + (size_t)slot * n,
                                                    S.nhat2 + (size_t)slot * n);
         } else {
-            bad |= v2::v2_square<LOGN, GLOBAL, WIDE>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv);
+            // In global mode with a two-chunk tile T stays in registers from the square to the
+            // reduction (run_bare keeps writing it to HBM, where mm_read_product finds it).
+            constexpr bool KEEP = REG_T && GLOBAL && !BARE && !WIDE && v2::Tiling<LOGN, TILED>::chunks == 2;
+            uint32_t keep[16];
+            bad |= v2::v2_square<LOGN, TILED, WIDE, 0, KEEP>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv, true, keep);
             if (!BARE)
-                bad |= v2::v2_redc<LOGN, GLOBAL, WIDE>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
-                                                       S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n);
+                bad |= v2::v2_redc<LOGN, TILED, WIDE, 0, KEEP>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n,
+                                                               S.nphat2 + (size_t)slot * n, S.nhat1 + (size_t)slot * n,
+                                                               S.nhat2 + (size_t)slot * n, true, keep);
         }
     }
     if (tid() == 0) {

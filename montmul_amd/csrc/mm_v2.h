@@ -388,11 +388,17 @@ template <u32 LOGN> struct CPlan {
 // longer vector is exchanged in chunks. At n = 8,192 the 1,024-thread workgroup has a CU to
 // itself, so its tile is 4,096 elements (96 KiB): two chunks, like n = 4,096, where four would
 // hold twice the registers in flight.
+//
+// TILED selects the exchange buffer: 0 holds the whole vector (LDS-resident layout), 1 the
+// default tile above, and a larger power of two that many elements (the autotuner's choice:
+// one 4,096-element tile at n = 4,096 exchanges in one chunk at one workgroup per CU).
 constexpr u32 tile_max(u32 logn) { return logn > 12 ? 4096 : 2048; }
-template <u32 LOGN, bool TILED> struct Tiling {
+template <u32 LOGN, u32 TILED> struct Tiling {
     static constexpr u32 n = 1u << LOGN;
-    static constexpr u32 tile = (TILED && n > tile_max(LOGN)) ? tile_max(LOGN) : n;
+    static constexpr u32 want = TILED == 0 ? n : TILED == 1 ? tile_max(LOGN) : TILED;
+    static constexpr u32 tile = n > want ? want : n;
     static constexpr u32 chunks = n / tile;
+    static_assert(TILED <= 1 || (TILED & (TILED - 1)) == 0, "a tile is a power of two");
 };
 
 // Moves the thread's slots from pass (ro, lso)'s layout to pass (ri, lsi)'s through the
@@ -400,7 +406,7 @@ template <u32 LOGN, bool TILED> struct Tiling {
 // land in separate registers until every chunk's outputs are written (a thread's input slot
 // can lie in an earlier chunk than the output slot held in the same register). Starts and
 // ends with the barriers that order it against the buffer's other uses.
-template <u32 LOGN, bool TILED>
+template <u32 LOGN, u32 TILED>
 __device__ __forceinline__ void exchange(u32 t, bool act, Z1* v1, Z2* v2, const Xbuf& X, u32 ro, u32 lso, u32 ri, u32 lsi) {
     typedef Tiling<LOGN, TILED> G;
     constexpr u32 T = CPlan<LOGN>::T;
@@ -503,7 +509,7 @@ __device__ __forceinline__ void exchange_wave(u32 t, bool act, Z1* v1, Z2* v2, c
 // One exchange between passes: inside the waves where it can be, else through the tile. A
 // wave-local exchange that follows a tile exchange (or starts a transform whose caller asks for
 // it) waits at a workgroup barrier first: the tile's last reads alias the waves' regions.
-template <u32 LOGN, bool TILED>
+template <u32 LOGN, u32 TILED>
 __device__ __forceinline__ void exchange_any(u32 t, bool act, Z1* v1, Z2* v2, const Xbuf& X, u32 ro, u32 lso, u32 ri, u32 lsi,
                                              bool& synced) {
     if (wave_local(ro, lso, ri, lsi)) {
@@ -531,7 +537,7 @@ template <u32 B, u32 E, class Fn> __device__ __forceinline__ void static_for(Fn&
 // threads when `on` is false (an idle slot group). GT: see slot_tid. SYNC: a barrier before a
 // first exchange that is wave-local (false when, since the last barrier, every wave has used
 // only its own region of the exchange buffer).
-template <u32 LOGN, bool TILED, u32 GT = 0, bool SYNC = true>
+template <u32 LOGN, u32 TILED, u32 GT = 0, bool SYNC = true>
 __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, const S2* tw2, const Xbuf& X, bool on = true) {
     typedef CPlan<LOGN> P;
     static_assert(16 * Tiling<LOGN, TILED>::tile >= 64 * P::T, "the wave regions fit the exchange buffer");
@@ -556,7 +562,7 @@ __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, c
 }
 
 // Full DIT from the last pass's layout back to pass 0's (natural positions). SYNC: as dif_regs.
-template <u32 LOGN, bool TILED, u32 GT = 0, bool SYNC = true>
+template <u32 LOGN, u32 TILED, u32 GT = 0, bool SYNC = true>
 __device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, const S2* itw2, const Xbuf& X, bool on = true) {
     typedef CPlan<LOGN> P;
     const bool act = on && t < P::T;

@@ -41,9 +41,11 @@ __host__ __device__ inline uint64_t v2_lds_resident(uint32_t n, uint32_t k, uint
 
 // One step's squaring: T = c x^2 into Tl (2k limbs); returns nonzero on a carry out.
 // iw1, iw2: the unweighting, zeta^-j / n or, on a 1 bit, zeta^-j c / n.
-template <u32 LOGN, bool TILED, bool WIDE = false, u32 GT = 0>
+// KEEP (exchange tile of n/2 elements, one slot per workgroup): T stays in registers instead,
+// thread t's limbs 8t .. 8t+7 in keep[0..7] and n + 8t .. n + 8t+7 in keep[8..15], for v2_redc.
+template <u32 LOGN, u32 TILED, bool WIDE = false, u32 GT = 0, bool KEEP = false>
 __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, u64* sc, const u32* x, const F1* iw1,
-                                         const S2* iw2, u32 inv, bool on = true) {
+                                         const S2* iw2, u32 inv, bool on = true, u32* keep = nullptr) {
     typedef CPlan<LOGN> P;
     u32 t = slot_tid<GT>();  // refreshed per phase: no address held across phases
     const u32 T = P::T, n = P::n;
@@ -72,6 +74,7 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
     typedef Tiling<LOGN, TILED> G;
     u32 l[16];
     u64 top;
+    static_assert(!KEEP || (G::chunks == 2 && !WIDE && GT == 0), "T in registers: two chunks, one slot per workgroup");
     if constexpr (G::chunks == 1) {
         const Cbuf C = cbuf(X, n);
         __syncthreads();
@@ -118,7 +121,12 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         const u64 mid = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) : (u128)0; }, lo, sc, 0);
         if (MM_ACT(t)) {
 #pragma unroll
-            for (u32 i = 0; i < 8; i++) Tl[8 * t + i] = lo[i];
+            for (u32 i = 0; i < 8; i++) {
+                if constexpr (KEEP)
+                    keep[i] = lo[i];
+                else
+                    Tl[8 * t + i] = lo[i];
+            }
 #pragma unroll
             for (u32 e = 0; e < 8; e++) cput(C, slot_pos(t, e, r0, ls0, T), crt_lazy(im1[e], im2[e], inv));
         }
@@ -128,7 +136,12 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
         const u64 top = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) : (u128)0; }, hi, sc, mid);
         if (MM_ACT(t)) {
 #pragma unroll
-            for (u32 i = 0; i < 8; i++) Tl[n + 8 * t + i] = hi[i];
+            for (u32 i = 0; i < 8; i++) {
+                if constexpr (KEEP)
+                    keep[8 + i] = hi[i];
+                else
+                    Tl[n + 8 * t + i] = hi[i];
+            }
         }
         return top != 0;
     }
@@ -142,9 +155,11 @@ __device__ __forceinline__ u32 v2_square(const VTab& V, const Xbuf& X, u32* Tl, 
 // Reduction: x = T R^-1 (lazy) from T in Tl, through m = (T mod R) N' mod R and the cyclic
 // product m N mod (R - 1) (montmul/src/cpu.rs:452-484, 297-328). Returns nonzero on a carry
 // that must vanish but does not.
-template <u32 LOGN, bool TILED, bool WIDE = false, u32 GT = 0>
+// KEEP: T comes from v2_square's registers (keep[16]); its low half reaches the transform's
+// positions through the exchange buffer instead of HBM.
+template <u32 LOGN, u32 TILED, bool WIDE = false, u32 GT = 0, bool KEEP = false>
 __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u64* sc, u32* x, const F1* np1, const F2* np2,
-                                       const F1* ab1, const F2* ab2, bool on = true) {
+                                       const F1* ab1, const F2* ab2, bool on = true, const u32* keep = nullptr) {
     const VTab& V = A.t;
     const VTab& H = A.ht;
     typedef CPlan<LOGN> P;
@@ -159,12 +174,22 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     const Cbuf C = cbuf(X, Tiling<LOGN, TILED>::tile);
     Z1 v1[8];
     Z2 v2[8];
+    const u32* Tsrc = Tl;
+    if constexpr (KEEP) {
+        // The low half of T through the exchange buffer (n words): written in limb order,
+        // read in the transform's; dif_regs orders its first use of the buffer with a barrier.
+        u32* Tx = (u32*)X.z2;
+        __syncthreads();  // the square's last reads of the buffer
+#pragma unroll
+        for (u32 i = 0; i < 8; i++) Tx[8 * t + i] = keep[i];
+        Tsrc = Tx;
+    }
     __syncthreads();  // T's limbs written by every thread
     if (MM_ACT(t)) {
 #pragma unroll
         for (u32 e = 0; e < 8; e++) {
             u32 pos = slot_pos(t, e, r0, ls0, T);
-            u32 tv = Tl[pos];
+            u32 tv = Tsrc[pos];
             v1[e] = Op1::real_mul(tv, V.w1[pos]);
             v2[e] = Op2::real_mul(tv, V.w2[pos]);
             MM_GROUP_FENCE(e);
@@ -256,7 +281,8 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     t = slot_tid<GT>();
     // G = (cyclic + T mod R) mod (R - 1), x = floor(T / R) + G.
     u32 xs[8];
-    u64 cc = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) + Tl[8 * t + i] : (u128)0; }, xs, sc, 0);
+    u64 cc = carry_regs<8, NT>(
+        t, [&](int i) { return MM_ACT(t) ? cget(C, 8 * t + i) + (KEEP ? keep[i] : Tl[8 * t + i]) : (u128)0; }, xs, sc, 0);
     // With several slot groups in the workgroup the fold count must be the same in all of them
     // (the carry holds barriers): they fold while any group has a carry left; a further fold
     // with carry 0 leaves a group's limbs as they are.
@@ -271,7 +297,8 @@ __device__ __forceinline__ u32 v2_redc(const VArgs& A, const Xbuf& X, u32* Tl, u
     u32 lo[8];
 #pragma unroll
     for (u32 i = 0; i < 8; i++) lo[i] = xs[i];
-    u64 top = carry_regs<8, NT>(t, [&](int i) { return MM_ACT(t) ? (u128)lo[i] + Tl[k + 8 * t + i] : (u128)0; }, xs, sc, 0);
+    u64 top = carry_regs<8, NT>(
+        t, [&](int i) { return MM_ACT(t) ? (u128)lo[i] + (KEEP ? keep[8 + i] : Tl[k + 8 * t + i]) : (u128)0; }, xs, sc, 0);
     t = slot_tid<GT>();
     if (MM_ACT(t)) {
 #pragma unroll

@@ -505,12 +505,13 @@ def serve_mix(server, make, moduli, base=3, timeout_s=120.0):
     return got
 
 
-def mixed_lengths(make, bit_sizes, ebits, chunk, seed=9):
+def mixed_lengths(make, bit_sizes, ebits, chunk, seed=9, ref=None):
     """A MixBatch over moduli of the given sizes (several transform lengths) against plain
-    per-length ModBatches on the same backend: every slot loaded with 1 or a random x, random
-    exponents of 0 .. ebits bits (so slots finish at different steps), run(chunk) until no bits
-    remain. Every slot: read == x^(2^nbits) 3^e mod N, and raw limbs equal to its length's
-    ModBatch. No reference counterpart (ModBatch holds one length, batch.rs:64-77)."""
+    per-length ModBatches, on the same backend or on `ref` (a factory, e.g. the oracle's closed
+    form): every slot loaded with 1 or a random x, random exponents of 0 .. ebits bits (so slots
+    finish at different steps), run(chunk) until no bits remain. Every slot: read ==
+    x^(2^nbits) 3^e mod N, and raw limbs equal to its length's ModBatch. No reference
+    counterpart (ModBatch holds one length, batch.rs:64-77)."""
     rng = random.Random(seed)
     moduli = [rand_odd(rng, b) for b in bit_sizes]
     xs, exps = [], []
@@ -534,7 +535,7 @@ def mixed_lengths(make, bit_sizes, ebits, chunk, seed=9):
         assert mix.read(i) == mulmod(powmod(xs[i], 1 << nb, n), powmod(3, exps[i], n), n), f"slot {i}, {n.bit_length()} bits"
     for length in lengths:
         idx = [i for i in range(len(moduli)) if mix.params(i).len == length]
-        plain = ModBatch(make(), [moduli[i] for i in idx], 3)
+        plain = ModBatch((ref or make)(), [moduli[i] for i in idx], 3)
         for j, i in enumerate(idx):
             plain.load(j, xs[i])
             plain.set_exponent(j, exps[i])

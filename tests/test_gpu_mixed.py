@@ -129,3 +129,14 @@ def test_mixed_1024_threads_with_radix4_global_length(hip_global):
     launch (whole-workgroup carries, a called function in the 1,024-thread build)."""
     lengths = common.mixed_lengths(lambda: hip_global, [150000, 300000, 5000, 60000], 24, 9, seed=12)
     assert lengths == [512, 4096, 16384, 32768]
+
+
+def test_mixed_raw_limbs_against_the_closed_form(hip_global):
+    """One persistent launch over L = 256 .. 16,384 (radix-4 slot groups, register kernels in
+    slot groups, wide and full workgroups), every slot's raw limbs against the oracle's Gmp
+    closed form with k = L/2 (gmp.rs:45-61) per length, not against another HIP batch."""
+    from oracle_backends import Gmp
+
+    sizes = [3000, 6000, 9000, 20000, 40000, 60000, 90000, 120000, 150000, 200000] * 2
+    lengths = common.mixed_lengths(lambda: hip_global, sizes, 24, 7, seed=21, ref=lambda: Gmp(full_k=True))
+    assert lengths == [256, 512, 1024, 2048, 4096, 8192, 16384]

@@ -200,6 +200,30 @@ def test_long_run_split_into_launches():
         h.close()
 
 
+def test_long_run_200000_steps():
+    """tests/gpu.rs:179-184 at its own length: 87,914-bit moduli and 200,000-bit exponents, one
+    run(200000) split into launches of about 200 ms (a plain first chunk, then graph replays),
+    against GMP powm on every slot (8 slots, checked on 8 host threads)."""
+    h = Hip(max_launch_ms=200)
+    try:
+        rng = random.Random(179)
+        moduli = [common.rand_odd(rng, 87_914) for _ in range(8)]
+        exps = [common.rand_exponent(rng, 200_000) for _ in range(8)]
+        b = ModBatch(h, moduli, 3)
+        for i, e in enumerate(exps):
+            b.load(i, 1)
+            b.set_exponent(i, e)
+        say("long run: 8 slots x 200,000 steps")
+        b.run(200_000)
+        ms, launches = Hip.last_run_kernel_ms(b)
+        say("long run:", launches, "launches,", round(ms), "ms; checking against powm")
+        assert launches > 1
+        assert b.check_pow_range(3, 0, exps, threads=8) is None
+        b.free()
+    finally:
+        h.close()
+
+
 def test_graph_replayed_chunks_match_eager():
     """With max_launch_ms, the whole chunks of a run after the first are replays of one captured
     graph. Several runs in a row, with host reads and eager runs of a second backend between
@@ -234,7 +258,8 @@ def test_graph_replayed_chunks_match_eager():
 
 def test_autotuned_plans_match_default():
     """The autotuner times every plan that fits (radix-8 register kernel or radix-4 kernel,
-    global or LDS-resident) on the batch's own slots and keeps the fastest: the chosen plan is
+    global or LDS-resident, exchange tile, slots per CU) on the batch's own slots and keeps the
+    fastest: the chosen plan is
     reported as tuned, a second batch of the same shape reuses it, and the results are the
     default planner's, bit for bit."""
     tuned = Hip(autotune=True)
@@ -253,6 +278,33 @@ def test_autotuned_plans_match_default():
             common.assert_same_raw(lambda: tuned, lambda: Hip(), bits, 16, 96, seed=bits)
     finally:
         tuned.close()
+
+
+def test_tile_and_slots_per_cu_plans_match_default():
+    """The plans the autotuner adds, fixed through the options: the 4,096-element exchange tile
+    at L = 8,192 (one chunk per exchange) and one slot per CU (LDS padded), alone and together,
+    give the default plan's raw limbs; a tile the batch's kernel cannot take is an error."""
+    for tile, spc in ((4096, 0), (0, 1), (4096, 1)):
+        say("plan", tile, spc)
+        h = Hip(tile=tile, slots_per_cu=spc)
+        try:
+            rng = random.Random(tile + spc)
+            b = ModBatch(h, [common.rand_odd(rng, 100_000) for _ in range(4)], 3)
+            p = Hip.plan_of(b)
+            assert p["global"] and p["kernel"] == 8 and p["tile"] == (tile or 2048) and p["slots_per_cu"] == spc, p
+            assert p["lds_bytes"] > (80 * 1024 if spc else 0)
+            b.free()
+            common.assert_same_raw(lambda: h, lambda: Hip(), 100_000, 6, 48, seed=tile + spc)
+            if spc and not tile:  # one slot per CU at a radix-4 length too
+                common.assert_same_raw(lambda: h, lambda: Hip(), 3000, 5, 40, seed=7)
+        finally:
+            h.close()
+    h = Hip(tile=4096)
+    try:
+        with pytest.raises(MontmulError, match="is not a plan for L = 2048"):
+            ModBatch(h, [common.rand_odd(random.Random(1), 20_000)], 3)
+    finally:
+        h.close()
 
 
 def test_two_batches_interleaved(hip):
@@ -350,6 +402,35 @@ def test_headline_all_4096_slots_against_powm(hip_global):
         assert b.check_pow_range(3, first, exps[first:first + 1024], threads=16) is None
     common.pow_range_check(lambda: hip_global, 3000, 9, 60)
     b.free()
+
+
+def test_full_fermat_20000_bits_every_slot(hip, hip_global):
+    """The full-test workload itself (config 3 of the survey's measurement table): 2,048 slots of
+    20,000-bit moduli (L = 2,048), the first 1,024 cycling through the planted primes of
+    tests/golden/planted_20000bit.txt, the rest from GMP's generator seeded 7, e = N - 1. Global
+    and LDS-resident plans give the same raw limbs on every slot; every planted prime reads 1,
+    and every other slot equals 3^(N-1) mod N (GMP powm on 16 host threads)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "planted_20000bit.txt")
+    planted = [int(line, 16) for line in open(path) if line.strip()]
+    rand, _ = mm.score_workload(20_000, 1024, 20_000, 7)
+    moduli = [planted[i % len(planted)] for i in range(1024)] + rand
+    exps = [n - 1 for n in moduli]
+    raws = []
+    for be, name in ((hip_global, "global"), (hip, "lds")):
+        say("full Fermat, 2,048 slots,", name)
+        b = ModBatch(be, moduli, 3)
+        assert b.params().len == 2048 and Hip.plan_of(b)["global"] == (name == "global")
+        for i, e in enumerate(exps):
+            b.load(i, 1)
+            b.set_exponent(i, e)
+        b.run(20_000)
+        raws.append([b.read_raw(i) for i in range(len(moduli))])
+        if name == "global":
+            assert all(b.is_one(i) for i in range(1024))
+            assert b.check_pow_range(3, 1024, exps[1024:], threads=16) is None
+        b.free()
+    bad = [i for i in range(len(moduli)) if raws[0][i] != raws[1][i]]
+    assert not bad, f"slots {bad[:8]} differ between the plans"
 
 
 def test_cpu_case_4096_bits(hip):
