@@ -116,9 +116,11 @@ typedef struct mm_options {
                               * LDS-resident, the register kernel's exchange tile, one or as many
                               * slots per CU as fit) for 16 steps on the batch's own slots and keep
                               * the fastest; the choice is cached per length and slot-count octave */
-    uint32_t tile;           /* without autotune: the register kernel's exchange tile in global
-                              * mode, in elements (0: the default, 2,048 up to L = 8,192 and 4,096
-                              * above; 4,096 is also built for L = 8,192) */
+    uint32_t tile;           /* without autotune, global mode: the register kernel's exchange tile
+                              * in elements (0: the default, 2,048 up to L = 8,192 and 4,096 above;
+                              * 4,096 is also built for L = 8,192), or for the radix-4 kernel 2,048
+                              * to run its narrow passes in an LDS tile at any length (by default
+                              * from L = 65,536) */
     uint32_t slots_per_cu;   /* without autotune: 1 pads each workgroup's LDS so that one slot runs
                               * per CU; 0: as many as fit */
 } mm_options;
@@ -131,7 +133,8 @@ typedef struct mm_plan_info {
     uint32_t slots_per_wg;   /* slots handled by one workgroup */
     uint32_t kernel;         /* 4: radix-4 step (LDS or global); 8: register-resident radix-8 step */
     int32_t tuned;           /* 1 when the autotuner chose this plan */
-    uint32_t tile;           /* register kernel in global mode: exchange tile in elements; else 0 */
+    uint32_t tile;           /* global mode: the LDS tile in elements (the register kernel's exchange
+                              * tile, or the radix-4 kernel's tile for its narrow passes); else 0 */
     uint32_t slots_per_cu;   /* 0: as many workgroups per CU as fit; 1: LDS padded to one per CU */
 } mm_plan_info;
 

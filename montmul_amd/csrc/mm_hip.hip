@@ -581,6 +581,7 @@ struct State {
     uint32_t k = 0, len = 0, n = 0, base = 0, slots = 0;
     bool global = false, wrap = true, v2 = false;
     uint32_t threads = 0, tile = 1, slots_per_cu = 0;
+    uint32_t v1_tile = 0;  // radix-4 kernel in global mode: its transforms' LDS tile (elements)
     size_t lds = 0;
     std::shared_ptr<DevTables> tab, half;
     // device buffers
@@ -630,6 +631,12 @@ struct State {
         A.iwc1 = d_iwc1;
         A.iwc2 = d_iwc2;
         A.coef = d_coef;
+        return A;
+    }
+    // The step kernel's arguments: args() plus the radix-4 kernel's LDS tile in global mode.
+    KArgs step_args() const {
+        KArgs A = args();
+        A.tile_m = !v2 && global ? v1_tile : 0;
         return A;
     }
     KArgs args() const {
@@ -901,7 +908,7 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
     // Only run_bare leaves each step's product in HBM: a full step of the register kernel keeps
     // it on chip where it can.
     S->product_in_hbm = bare;
-    KArgs A = S->args();
+    KArgs A = S->step_args();
     StepKernel kern = pick_step(S->global, S->wrap, bare);
     v2::VArgs A2{};
     StepKernelV2 kern2 = nullptr;
@@ -945,19 +952,32 @@ int64_t steps(State* S, uint64_t bits, bool bare) {
 }
 
 // ---- Plans and the autotuner ------------------------------------------------------------------
+// The radix-4 kernel's LDS tile in global mode, in elements of one field (32 KiB of F2): its
+// transforms run the passes that act within 2,048 elements there (lib.rs:455-511). By default
+// from n = 32,768 (L = 65,536) on, where the vector's in-place passes miss L2: +10 % at
+// L = 65,536 and +19 % at 131,072, but -16 % at L = 8,192, where they hit it and the tile's
+// power-of-two strides conflict in LDS banks. mm_options.tile = 2,048 asks for it at any length.
+constexpr uint32_t V1_TILE = 2048, V1_TILE_MIN_N = 32768;
+uint32_t v1_tile_for(uint32_t n, bool global, uint32_t tile) {
+    if (!global) return 0;
+    return tile == V1_TILE || n >= V1_TILE_MIN_N ? std::min<uint32_t>(n, V1_TILE) : 0;
+}
+
 // Dynamic LDS at which a second workgroup no longer fits a CU beside the first.
 uint64_t one_per_cu_lds(const Ctx* C) { return C->max_lds / 2 + 2048; }
 
 // The launch shape of a plan: the register kernel (n/8 threads; an exchange tile in global mode,
 // the whole vector and T resident otherwise) or the radix-4 kernel (one radix-4 group per
-// thread; the vector resident in LDS, or only the carry scratch in global mode). With one slot
-// per CU the workgroup's dynamic LDS is padded past half of a CU's, so no second one fits.
+// thread; the vector resident in LDS, or in HBM with the carry scratch and a tile for the
+// transforms' narrow passes in LDS). With one slot per CU the workgroup's dynamic LDS is padded
+// past half of a CU's, so no second one fits.
 void set_plan(State* S, const PlanChoice& p) {
     const uint32_t n = S->n, k = S->k;
     S->v2 = p.v2;
     S->global = p.global;
-    S->tile = p.v2 && p.global ? p.tile : 1;
+    S->tile = p.global ? p.tile : 1;
     S->slots_per_cu = p.slots_per_cu;
+    S->v1_tile = 0;
     if (p.v2) {
         S->threads = std::min<uint32_t>(n / 8, 1024);  // n = 2^14: two halves of n/2, n/16 threads
         const uint64_t want = S->tile != 1 ? S->tile : (n > 4096 ? 4096 : 2048);  // v2::Tiling
@@ -965,7 +985,8 @@ void set_plan(State* S, const PlanChoice& p) {
         S->lds = p.global ? 24 * tile + lds_scratch_bytes(S->threads) : v2::v2_lds_resident(n, k, S->threads);
     } else {
         S->threads = plan_threads(n);
-        S->lds = p.global ? lds_scratch_bytes(S->threads) : lds_bytes_resident(n, k, S->threads);
+        S->v1_tile = v1_tile_for(n, p.global, p.tile);
+        S->lds = p.global ? v1_tile_offset(S->threads) + 16ull * S->v1_tile : lds_bytes_resident(n, k, S->threads);
     }
     if (p.slots_per_cu == 1) S->lds = std::max<size_t>(S->lds, one_per_cu_lds(S->C));
 }
@@ -984,14 +1005,15 @@ bool plan_fits(const State* S, const PlanChoice& p) {
     if (p.v2 && !(S->wrap && S->n >= 512 && S->n <= 16384)) return false;
     if (p.v2 && S->n > 8192 && !p.global) return false;  // n = 2^14: global mode only
     if (p.v2 && !v2_tile_ok(S->n, p.global, p.tile)) return false;
-    if (!p.v2 && p.tile != 1) return false;
+    if (!p.v2 && p.tile != 1 && !(p.global && p.tile == V1_TILE)) return false;
     uint64_t lds;
     if (p.v2) {
         const uint64_t want = p.tile != 1 ? p.tile : (S->n > 4096 ? 4096 : 2048);
         lds = p.global ? 24 * std::min<uint64_t>(S->n, want) + lds_scratch_bytes(std::min<uint32_t>(S->n / 8, 1024))
                        : v2::v2_lds_resident(S->n, S->k, S->n / 8);
     } else {
-        lds = p.global ? lds_scratch_bytes(plan_threads(S->n)) : lds_bytes_resident(S->n, S->k, plan_threads(S->n));
+        lds = p.global ? v1_tile_offset(plan_threads(S->n)) + 16ull * v1_tile_for(S->n, true, p.tile)
+                       : lds_bytes_resident(S->n, S->k, plan_threads(S->n));
     }
     if (p.slots_per_cu == 1) lds = std::max<uint64_t>(lds, one_per_cu_lds(C));
     return lds <= (p.v2 ? C->max_dyn_lds_v2 : C->max_dyn_lds);
@@ -1032,7 +1054,7 @@ int autotune(State* S, bool v2ok) {
         std::vector<PlanChoice> cands;
         for (int v = v2ok ? 1 : 0; v >= 0; v--)
             for (int g = 1; g >= 0; g--)
-                for (uint32_t tile : {1u, V2_WIDE_TILE})
+                for (uint32_t tile : {1u, V2_WIDE_TILE, V1_TILE})
                     for (uint32_t spc : {0u, 1u}) {
                         PlanChoice p;
                         p.v2 = v;
@@ -1052,7 +1074,7 @@ int autotune(State* S, bool v2ok) {
                                        S->lds, sm, S->vargs(), 0u, T);
                 else
                     hipLaunchKernelGGL(pick_step(p.global, S->wrap, false), dim3(S->slots), dim3(S->threads), S->lds, sm,
-                                       S->args(), 0u, T);
+                                       S->step_args(), 0u, T);
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipEventRecord(C->ev1, sm));
                 HIPCHK(hipEventSynchronize(C->ev1));
@@ -1117,7 +1139,8 @@ int mix_class(State* S, uint32_t nt, MixClass* out, uint64_t* lds, uint32_t* gro
         return 1;
     out->kind = MIX_V1 | (global ? MIX_V1_GLOBAL : 0) | (S->wrap ? MIX_V1_WRAP : 0);
     out->k = S->args();
-    *lds = global ? lds_scratch_bytes(nt) : resident;
+    out->k.tile_m = v1_tile_for(n, global, 1);  // its transforms' LDS tile, where it pays
+    *lds = global ? v1_tile_offset(nt) + 16ull * out->k.tile_m : resident;
     return 0;
 }
 
@@ -1637,7 +1660,9 @@ int mm_plan(void* st, mm_plan_info* out) {
     out->slots_per_wg = 1;
     out->kernel = S->v2 ? 8 : 4;
     out->tuned = S->tuned ? 1 : 0;
-    out->tile = S->v2 && S->global ? (S->tile != 1 ? S->tile : std::min<uint32_t>(S->n, S->n > 4096 ? 4096 : 2048)) : 0;
+    out->tile = !S->global ? 0
+                : S->v2       ? (S->tile != 1 ? S->tile : std::min<uint32_t>(S->n, S->n > 4096 ? 4096 : 2048))
+                              : S->v1_tile;
     out->slots_per_cu = S->slots_per_cu;
     return 0;
 }

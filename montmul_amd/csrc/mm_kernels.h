@@ -137,55 +137,120 @@ __device__ __forceinline__ void dit4(F* a, uint32_t i0, uint32_t q, const F* tw,
 __host__ __device__ __forceinline__ uint32_t dif_pass_q(uint32_t logn, uint32_t p) { return 1u << (logn - 2 - 2 * p); }
 __host__ __device__ __forceinline__ uint32_t dit_pass_q(uint32_t logn, uint32_t p) { return 1u << ((logn & 1) + 2 * p); }
 
+// One radix-4 DIF pass of stride q over a[0, len) (len a multiple of 4q), twiddles from offset
+// `off` (in triples).
+template <class F, class Geo>
+__device__ __forceinline__ void dif_pass(F* a, uint32_t len, uint32_t q, uint32_t lq, const F* tw, bool plus) {
+    for (uint32_t b = Geo::t(); b < len / 4; b += Geo::n()) {
+        uint32_t j = b & (q - 1);
+        uint32_t i0 = ((b >> lq) << (lq + 2)) + j;
+        dif4(a, i0, q, tw + 3 * j, q == 1, plus);
+    }
+}
+template <class F, class Geo>
+__device__ __forceinline__ void dit_pass(F* a, uint32_t len, uint32_t q, uint32_t lq, const F* tw, bool plus) {
+    for (uint32_t b = Geo::t(); b < len / 4; b += Geo::n()) {
+        uint32_t j = b & (q - 1);
+        uint32_t i0 = ((b >> lq) << (lq + 2)) + j;
+        dit4(a, i0, q, tw + 3 * j, q == 1, plus);
+    }
+}
+template <class F, class Geo>
+__device__ __forceinline__ void radix2_pass(F* a, uint32_t len) {
+    for (uint32_t b = Geo::t(); b < len / 2; b += Geo::n()) {
+        F u = a[2 * b], v = a[2 * b + 1];
+        a[2 * b] = cadd(u, v);
+        a[2 * b + 1] = csub(u, v);
+    }
+}
+
 // Full DIF over a (length t.n). Ends with a barrier.
+//
+// With a tile (m elements of LDS, a power of two, a in global memory: lib.rs:455-511), the passes
+// whose butterfly groups span more than m elements run in place, and the rest, which act within
+// aligned blocks of at most m, run block after block in the tile: one global round trip for
+// them instead of one per pass. Each thread copies back exactly the elements it copied in, so
+// the next block's copy-in needs no barrier. A pass's twiddles depend only on the position in
+// its butterfly group, so a block uses the same twiddle rows as the whole vector.
 template <class F, class Geo = WholeBlock>
-__device__ void dif(F* a, const Tab& t, const F* dtw) {
+__device__ void dif(F* a, const Tab& t, const F* dtw, F* tile = nullptr, uint32_t m = 0) {
+    if (m == 0) tile = nullptr;  // no tile
     const uint32_t n = t.n, logn = t.logn, npass = logn / 2;
     const bool plus = t.i_is_plus;
-    uint32_t off = 0;
-    for (uint32_t p = 0; p < npass; p++) {
+    uint32_t off = 0, p = 0;
+    for (; p < npass; p++) {
         const uint32_t q = dif_pass_q(logn, p), lq = logn - 2 - 2 * p;
+        if (tile && 4 * q <= m) break;
         __syncthreads();
-        for (uint32_t b = Geo::t(); b < n / 4; b += Geo::n()) {
-            uint32_t j = b & (q - 1);
-            uint32_t i0 = ((b >> lq) << (lq + 2)) + j;
-            dif4(a, i0, q, dtw + 3 * (off + j), q == 1, plus);
-        }
+        dif_pass<F, Geo>(a, n, q, lq, dtw + 3 * off, plus);
         off += q;
     }
-    if (logn & 1) {
-        __syncthreads();
-        for (uint32_t b = Geo::t(); b < n / 2; b += Geo::n()) {
-            F u = a[2 * b], v = a[2 * b + 1];
-            a[2 * b] = cadd(u, v);
-            a[2 * b + 1] = csub(u, v);
+    if (tile && (p < npass || (logn & 1))) {
+        const uint32_t len = m < n ? m : n;
+        for (uint32_t c = 0; c < n; c += len) {
+            if (c == 0) __syncthreads();  // the in-place passes are done
+            for (uint32_t j = Geo::t(); j < len; j += Geo::n()) tile[j] = a[c + j];
+            uint32_t o = off;
+            for (uint32_t pp = p; pp < npass; pp++) {
+                const uint32_t q = dif_pass_q(logn, pp), lq = logn - 2 - 2 * pp;
+                __syncthreads();
+                dif_pass<F, Geo>(tile, len, q, lq, dtw + 3 * o, plus);
+                o += q;
+            }
+            if (logn & 1) {
+                __syncthreads();
+                radix2_pass<F, Geo>(tile, len);
+            }
+            __syncthreads();
+            for (uint32_t j = Geo::t(); j < len; j += Geo::n()) a[c + j] = tile[j];
         }
+    } else if (logn & 1) {
+        __syncthreads();
+        radix2_pass<F, Geo>(a, n);
     }
     __syncthreads();
 }
 
-// Full DIT over a (length t.n), unscaled. Ends with a barrier.
+// Full DIT over a (length t.n), unscaled. Ends with a barrier. With a tile: the mirror image
+// of dif's, the passes within blocks of at most m first, block after block in the tile, then
+// the wider passes in place.
 template <class F, class Geo = WholeBlock>
-__device__ void dit(F* a, const Tab& t, const F* itw) {
+__device__ void dit(F* a, const Tab& t, const F* itw, F* tile = nullptr, uint32_t m = 0) {
+    if (m == 0) tile = nullptr;  // no tile
     const uint32_t n = t.n, logn = t.logn, npass = logn / 2;
     const bool plus = t.i_is_plus;
-    if (logn & 1) {
-        __syncthreads();
-        for (uint32_t b = Geo::t(); b < n / 2; b += Geo::n()) {
-            F u = a[2 * b], v = a[2 * b + 1];
-            a[2 * b] = cadd(u, v);
-            a[2 * b + 1] = csub(u, v);
+    uint32_t off = 0, p = 0;
+    if (tile) {
+        const uint32_t len = m < n ? m : n;
+        uint32_t np = 0;  // passes that act within blocks of len
+        while (np < npass && 4 * dit_pass_q(logn, np) <= len) np++;
+        for (uint32_t c = 0; c < n; c += len) {
+            if (c == 0) __syncthreads();  // the vector's last writes (by any thread) are done
+            for (uint32_t j = Geo::t(); j < len; j += Geo::n()) tile[j] = a[c + j];
+            if (logn & 1) {
+                __syncthreads();
+                radix2_pass<F, Geo>(tile, len);
+            }
+            uint32_t o = 0;
+            for (uint32_t pp = 0; pp < np; pp++) {
+                const uint32_t q = dit_pass_q(logn, pp), lq = (logn & 1) + 2 * pp;
+                __syncthreads();
+                dit_pass<F, Geo>(tile, len, q, lq, itw + 3 * o, plus);
+                o += q;
+            }
+            __syncthreads();
+            for (uint32_t j = Geo::t(); j < len; j += Geo::n()) a[c + j] = tile[j];
         }
+        p = np;
+        for (uint32_t pp = 0; pp < np; pp++) off += dit_pass_q(logn, pp);
+    } else if (logn & 1) {
+        __syncthreads();
+        radix2_pass<F, Geo>(a, n);
     }
-    uint32_t off = 0;
-    for (uint32_t p = 0; p < npass; p++) {
+    for (; p < npass; p++) {
         const uint32_t q = dit_pass_q(logn, p), lq = (logn & 1) + 2 * p;
         __syncthreads();
-        for (uint32_t b = Geo::t(); b < n / 4; b += Geo::n()) {
-            uint32_t j = b & (q - 1);
-            uint32_t i0 = ((b >> lq) << (lq + 2)) + j;
-            dit4(a, i0, q, itw + 3 * (off + j), q == 1, plus);
-        }
+        dit_pass<F, Geo>(a, n, q, lq, itw + 3 * off, plus);
         off += q;
     }
     __syncthreads();

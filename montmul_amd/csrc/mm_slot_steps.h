@@ -13,7 +13,12 @@ struct KArgs {
     Tab t;
     HalfTab ht;
     Slots s;
+    uint32_t tile_m;  // global mode: LDS tile of the radix-4 transforms, in elements (0: none)
 };
+
+// LDS of the radix-4 step in global mode: the carry scratch, then (16-byte aligned) a tile of
+// tile_m elements of one field at a time.
+__host__ __device__ inline uint32_t v1_tile_offset(uint32_t threads) { return (lds_scratch_bytes(threads) + 15) & ~15u; }
 
 // The register-transform kernels are held to 128 VGPRs: four waves per SIMD.
 #ifndef MM_V2_WAVES_PER_EU
@@ -33,6 +38,8 @@ __device__ __forceinline__ void steps_v1(const KArgs& A, uint32_t slot, uint32_t
         w.lt = S.lim + (size_t)slot * 3 * k;
         w.ls = w.lt + 2 * k;
         w.sc = (uint64_t*)lds;
+        w.tile = lds + v1_tile_offset(blockDim.x);
+        w.tile_m = A.tile_m;
     } else {
         w.a2 = (F2*)lds;
         w.a1 = (F1*)(lds + 16 * (size_t)n);

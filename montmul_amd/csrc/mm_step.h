@@ -17,6 +17,12 @@ struct Work {
     uint32_t* lt;
     uint32_t* ls;
     uint64_t* sc;
+    // Global mode: an LDS tile of tile_m elements (16 tile_m bytes, one field at a time) for the
+    // transforms' narrow passes (dif/dit); none when tile_m is 0.
+    unsigned char* tile = nullptr;
+    uint32_t tile_m = 0;
+    __device__ __forceinline__ F1* tile1() const { return (F1*)tile; }
+    __device__ __forceinline__ F2* tile2() const { return (F2*)tile; }
 };
 
 // LDS layout of LDS mode: F2[n] | F1[n] | T: u32[2k] | u32[k] | scratch. Global mode keeps
@@ -30,8 +36,8 @@ __host__ __device__ inline uint64_t lds_bytes_resident(uint32_t n, uint32_t k, u
 template <class Geo = WholeBlock>
 __device__ uint64_t square(const Work& w, const uint32_t* x, uint32_t k, uint32_t c, const Tab& t) {
     load_weighted<Geo>(w.a1, w.a2, x, k, t);
-    dif<F1, Geo>(w.a1, t, t.dtw1);
-    dif<F2, Geo>(w.a2, t, t.dtw2);
+    dif<F1, Geo>(w.a1, t, t.dtw1, w.tile1(), w.tile_m);
+    dif<F2, Geo>(w.a2, t, t.dtw2, w.tile2(), w.tile_m);
     for (uint32_t j = Geo::t(); j < t.n; j += Geo::n()) {
         F1 v1 = csqr(w.a1[j]);
         F2 v2 = csqr(w.a2[j]);
@@ -42,8 +48,8 @@ __device__ uint64_t square(const Work& w, const uint32_t* x, uint32_t k, uint32_
         w.a1[j] = v1;
         w.a2[j] = v2;
     }
-    dit<F1, Geo>(w.a1, t, t.itw1);
-    dit<F2, Geo>(w.a2, t, t.itw2);
+    dit<F1, Geo>(w.a1, t, t.itw1, w.tile1(), w.tile_m);
+    dit<F2, Geo>(w.a2, t, t.itw2, w.tile2(), w.tile_m);
     unweight<Geo>(w.a1, w.a2, t);
     __syncthreads();
     return block_carry<CARRY_E, ProductCoef, StoreLimbs, Geo>(ProductCoef{w.a1, w.a2, t.n, t.crt_inv}, 2 * k, StoreLimbs{w.lt}, w.sc, 0);
@@ -138,11 +144,11 @@ __device__ uint32_t redc(const Work& w, uint32_t* x, uint32_t k, const Tab& t, c
     static_assert(WRAP || !Geo::grouped, "slot groups need the wraparound reduction");
     // m = (T mod R) N' mod R
     load_weighted<Geo>(w.a1, w.a2, w.lt, k, t);
-    dif<F1, Geo>(w.a1, t, t.dtw1);
-    dif<F2, Geo>(w.a2, t, t.dtw2);
+    dif<F1, Geo>(w.a1, t, t.dtw1, w.tile1(), w.tile_m);
+    dif<F2, Geo>(w.a2, t, t.dtw2, w.tile2(), w.tile_m);
     pointwise_const<Geo>(w, nphat1, nphat2, t.n);
-    dit<F1, Geo>(w.a1, t, t.itw1);
-    dit<F2, Geo>(w.a2, t, t.itw2);
+    dit<F1, Geo>(w.a1, t, t.itw1, w.tile1(), w.tile_m);
+    dit<F2, Geo>(w.a2, t, t.itw2, w.tile2(), w.tile_m);
     unweight<Geo>(w.a1, w.a2, t);
     __syncthreads();
     if (WRAP) {
@@ -152,14 +158,14 @@ __device__ uint32_t redc(const Work& w, uint32_t* x, uint32_t k, const Tab& t, c
                                                             w.sc, 0);
         // The packed spectrum occupies elements [0, h), the product goes to [h, 2h = n).
         const uint32_t h = ht.t.n;
-        dif<F1, Geo>(w.a1, ht.t, ht.t.dtw1);
-        dif<F2, Geo>(w.a2, ht.t, ht.t.dtw2);
+        dif<F1, Geo>(w.a1, ht.t, ht.t.dtw1, w.tile1(), w.tile_m);
+        dif<F2, Geo>(w.a2, ht.t, ht.t.dtw2, w.tile2(), w.tile_m);
         F1* b1 = w.a1 + h;
         F2* b2 = w.a2 + h;
         half_pointwise<F1, Geo>(w.a1, b1, nhat1, h);
         half_pointwise<F2, Geo>(w.a2, b2, nhat2, h);
-        dit<F1, Geo>(b1, ht.t, ht.t.itw1);
-        dit<F2, Geo>(b2, ht.t, ht.t.itw2);
+        dit<F1, Geo>(b1, ht.t, ht.t.itw1, w.tile1(), w.tile_m);
+        dit<F2, Geo>(b2, ht.t, ht.t.itw2, w.tile2(), w.tile_m);
         // G = (cyclic + T mod R) mod (R - 1), then x = floor(T / R) + G.
         uint64_t c = block_carry<CARRY_E, CyclicCoef, StoreLimbs, Geo>(CyclicCoef{b1, b2, w.lt, t.crt_inv}, k, StoreLimbs{x}, w.sc, 0);
         // End-around carry: 2^(32k) = 1 mod R - 1. One fold in practice; bounded by the
@@ -176,11 +182,11 @@ __device__ uint32_t redc(const Work& w, uint32_t* x, uint32_t k, const Tab& t, c
         block_carry<CARRY_E>(ProductCoef{w.a1, w.a2, t.n, t.crt_inv}, k, StoreLimbs{w.ls}, w.sc, 0);
         // S = m N
         load_weighted(w.a1, w.a2, w.ls, k, t);
-        dif(w.a1, t, t.dtw1);
-        dif(w.a2, t, t.dtw2);
+        dif(w.a1, t, t.dtw1, w.tile1(), w.tile_m);
+        dif(w.a2, t, t.dtw2, w.tile2(), w.tile_m);
         pointwise_const(w, nhat1, nhat2, t.n);
-        dit(w.a1, t, t.itw1);
-        dit(w.a2, t, t.itw2);
+        dit(w.a1, t, t.itw1, w.tile1(), w.tile_m);
+        dit(w.a2, t, t.itw2, w.tile2(), w.tile_m);
         unweight(w.a1, w.a2, t);
         __syncthreads();
         uint64_t top_s = block_carry<CARRY_E>(ProductCoef{w.a1, w.a2, t.n, t.crt_inv}, 2 * k, StoreTop{w.ls, k}, w.sc, 0);

@@ -93,6 +93,30 @@ def test_large_lengths_global(hip):
         common.assert_same_raw(lambda: hip, lambda: Cpu(wraparound=True), bits, 2, 6)
 
 
+def test_radix4_global_tiles_every_length(monkeypatch, hip_global):
+    """The radix-4 kernel in global mode (forced with MONTMUL_KERNEL=v1) runs its transforms'
+    narrow passes in an LDS tile of up to 2,048 elements (lib.rs:455-511): by default from
+    L = 65,536 on, and at any length with tile = 2,048 (one tile for the whole vector up to
+    n = 2,048, 2 to 32 tiles above). L = 8 .. 131,072 (odd and even log2 n), tiles forced: raw
+    limbs equal the oracle's Gmp closed form with k = L/2 after 12 steps on 2 slots."""
+    monkeypatch.setenv("MONTMUL_KERNEL", "v1")
+    tiled = Hip(force_global=True, tile=2048)
+    try:
+        for bits in (64, 200, 500, 1000, 2000, 4000, 8000, 12_000, 20_000, 40_000, 90_000, 150_000, 300_000, 1_000_000,
+                     2_000_000):
+            moduli = [common.rand_odd(random.Random(bits), bits)]
+            b, d = ModBatch(tiled, moduli, 3), ModBatch(hip_global, moduli, 3)
+            p, n = Hip.plan_of(b), b.params().len // 2
+            assert p["kernel"] == 4 and p["global"] and p["tile"] == min(n, 2048), (bits, p)
+            assert Hip.plan_of(d)["tile"] == (2048 if n >= 32768 else 0), (bits, Hip.plan_of(d))
+            b.free()
+            d.free()
+            say("radix-4 tiles, L =", 2 * n)
+            common.assert_same_raw(lambda: tiled, lambda: Gmp(full_k=True), bits, 2, 12, seed=bits)
+    finally:
+        tiled.close()
+
+
 def test_random_modmul(hip):
     for bits, cases in [(64, 100), (1000, 60), (10_000, 16), (100_000, 4)]:
         for base in (3, 4):
