@@ -157,9 +157,11 @@ def candidates(hip):
 
 def rocprof_metrics(slots):
     """The --ncu analogue (main.rs:218-250): one 16-step launch of the score workload under
-    rocprofv3, giving the step kernel's VALU issue share (SQ_INSTS_VALU over the CUs' cycles,
-    GRBM_GUI_ACTIVE summed over 8 XCDs) and its HBM traffic rate (FETCH_SIZE x 2, the scale of a
-    plain copy measured on this GPU, plus WRITE_SIZE, over 8 TB/s)."""
+    rocprofv3, giving the step kernel's VALU issue (SQ_INSTS_VALU over the CUs' cycles,
+    GRBM_GUI_ACTIVE summed over 8 XCDs, in percent of one wave instruction per CU per cycle; it
+    can pass 100, as gfx950 issues simple 32-bit operations at up to about 1.37 per cycle) and
+    its HBM traffic rate (FETCH_SIZE x 2, the scale of a plain copy measured on this GPU, plus
+    WRITE_SIZE, over 8 TB/s)."""
     prof = shutil.which("rocprofv3")
     if not prof:
         raise RuntimeError("rocprofv3 not found")

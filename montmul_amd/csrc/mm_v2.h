@@ -1,17 +1,21 @@
-// mm_v2.h - register-resident transforms for the fused step (lengths n = 512 .. 8,192).
+// mm_v2.h - register-resident transforms for the fused step (lengths n = 512 .. 8,192, and the
+// two halves of n = 16,384).
 //
 // Each thread holds 8 elements of both fields in registers; a transform is a sequence of
 // radix-8 (or one radix-2/4) passes, each a butterfly over the thread's elements followed (DIF)
-// or preceded (DIT) by that pass's twiddles, with an exchange through a swizzled buffer
-// (LDS, or the slot's HBM work buffer in global mode) between passes. The pass layouts make
+// or preceded (DIT) by that pass's twiddles, with an exchange through a swizzled LDS buffer
+// between passes (the whole vector, or in global mode a tile of it; an exchange whose partners
+// lie in one wave goes through that wave's own part of the buffer). The pass layouts make
 // the pointwise steps exchange-free: after the last DIF pass thread t holds positions
 // 8t .. 8t+7, which is also where the first DIT pass reads.
 //
 // Arithmetic is lazy. GF(M61) values are 64-bit words <= 2^61 + 6 (a fold (x & P2) + (x >> 61)
 // after every operation keeps them there); GF(M31) values are 32-bit words in [0, P1], P1
-// standing for 0. Products go through four v_mad_u64_u32 into 128 bits and one Mersenne
-// fold; on gfx950 these multiplies issue at the rate of an add, so instruction count is the
-// cost. Constants read from memory (twiddles, weights, spectra) are canonical.
+// standing for 0. Products go through 32 x 32 -> 64 multiply-adds (v_mad_u64_u32) on 31-bit
+// halves and one Mersenne fold. On gfx950 a multiply-add issues at about the rate of a 64-bit
+// add (0.93 against 0.96 per CU per cycle; 32-bit adds 1.37), so the count of 64-bit
+// instructions, not of multiplies, is the cost. Constants read from memory (twiddles, weights,
+// spectra) are canonical.
 //
 // The roots are those of the reference (field.rs:237-267): in both fields omega^(n/4) = i and
 // omega^(n/8) = 2^h (1 + i) with h = 30 (M61) and 15 (M31), so the radix-8 butterflies need
