@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--ebits", type=int, default=2000)
     ap.add_argument("--sweep-ebits", type=int, default=200)
     ap.add_argument("--modes", default="global,lds")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="global mode: the exchange tile in elements (0: the default plan's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check-all", action="store_true",
                     help="skip the check of every headline slot against GMP's powm (about 3 minutes on 16 threads)")
@@ -224,7 +226,7 @@ def main():
     modes = [m for m in args.modes.split(",") if m]
     results = {}
     for mode in modes:
-        hip = mm.Hip(force_global=(mode == "global"), prefer_lds=(mode == "lds"))
+        hip = mm.Hip(force_global=(mode == "global"), prefer_lds=(mode == "lds"), tile=args.tile if mode == "global" else 0)
         cus = hip.cu_count()
         if args.slots:
             best, sw = args.slots, {}
@@ -290,7 +292,8 @@ def main():
                                 if "global" in results else f"{modes[0]} mode"),
                    "bits": args.bits, "L": L, "k": head["k"], "slots": slots, "ebits": ebits,
                    "mode": "global" if "global" in results else modes[0],
-                   "threads_per_wg": head["plan"]["threads"], "lds_bytes": head["plan"]["lds_bytes"]},
+                   "threads_per_wg": head["plan"]["threads"], "lds_bytes": head["plan"]["lds_bytes"],
+                   "tile": head["plan"]["tile"]},
         # The same passes with the upload of x and the exponents (host to device) inside the
         # clock, as the reference's score times run() (score.rs:98-102).
         "value_incl_upload": round(total / (sum(head["secs"]) + sum(head["ups"])), 1),

@@ -117,10 +117,11 @@ typedef struct mm_options {
                               * slots per CU as fit) for 16 steps on the batch's own slots and keep
                               * the fastest; the choice is cached per length and slot-count octave */
     uint32_t tile;           /* without autotune, global mode: the register kernel's exchange tile
-                              * in elements (0: the default, 2,048 up to L = 8,192 and 4,096 above;
-                              * 4,096 is also built for L = 8,192), or for the radix-4 kernel 2,048
-                              * to run its narrow passes in an LDS tile at any length (by default
-                              * from L = 65,536) */
+                              * in elements (0: the default, half the vector at L = 1,024 .. 4,096,
+                              * 2,048 at L = 8,192 and 4,096 at 16,384; also built: the whole
+                              * vector at L = 1,024 .. 4,096 and 4,096 at L = 8,192), or for the
+                              * radix-4 kernel 2,048 to run its narrow passes in an LDS tile at
+                              * any length (by default from L = 65,536) */
     uint32_t slots_per_cu;   /* without autotune: 1 pads each workgroup's LDS so that one slot runs
                               * per CU; 0: as many as fit */
 } mm_options;

@@ -672,14 +672,32 @@ template <uint32_t LOGN> StepKernelV2 pick_v2(bool global, bool bare) {
     if (global) return bare ? k_steps_v2<LOGN, true, true> : k_steps_v2<LOGN, true, false>;
     return bare ? k_steps_v2<LOGN, false, true> : k_steps_v2<LOGN, false, false>;
 }
-// The exchange tiles the autotuner may pick besides the default (v2::tile_max): one tile of the
-// whole vector at n = 4,096 in global mode (one chunk per exchange, one workgroup per CU).
+// The exchange tiles the autotuner may pick besides the default (v2::tile_max), in global mode:
+// one tile of the whole vector at n = 4,096 (one chunk per exchange, one workgroup per CU), and
+// half the vector at n = 512 .. 2,048 (two chunks per exchange, half the LDS per workgroup).
 constexpr uint32_t V2_WIDE_TILE = 4096;
-bool v2_tile_ok(uint32_t n, bool global, uint32_t tile) { return tile == 1 || (global && tile == V2_WIDE_TILE && n == 4096); }
+bool v2_tile_ok(uint32_t n, bool global, uint32_t tile) {
+    return tile == 1 || (global && ((tile == V2_WIDE_TILE && n == 4096) || (tile == n / 2 && n >= 512 && n <= 2048)));
+}
+// A tile asked for by its size: the size of the kernel's built-in tile (v2::tile_max, or the
+// whole vector below it) is that tile, 1.
+uint32_t v2_tile_norm(uint32_t n, uint32_t tile) { return tile == std::min<uint32_t>(n, n > 4096 ? 4096 : 2048) ? 1 : tile; }
+// The register kernel's exchange tile when none is asked for: in global mode at n = 512 .. 2,048
+// half the vector, whose halved LDS lets a CU hold as many workgroups as its registers allow
+// (four waves per SIMD rather than three) and keeps T in registers; the two chunks per exchange
+// cost less than that gains (at 4,096 slots: L = 1,024 +22 %, 2,048 +18 %, 4,096 +8 %; at L =
+// 2,048 with 2,048 slots, which then all fit at once, +35 %). Else the built-in tile.
+uint32_t v2_default_tile(uint32_t n, bool global) { return global && n >= 512 && n <= 2048 ? n / 2 : 1; }
+template <uint32_t LOGN> StepKernelV2 pick_half_v2(bool bare) {
+    return bare ? k_steps_v2<LOGN, true, true, (1u << LOGN) / 2> : k_steps_v2<LOGN, true, false, (1u << LOGN) / 2>;
+}
 StepKernelV2 pick_step_v2(uint32_t logn, bool global, bool bare, uint32_t tile = 1) {
     if (tile != 1) {
-        if (logn == 12 && global && tile == V2_WIDE_TILE)
-            return bare ? k_steps_v2<12, true, true, V2_WIDE_TILE> : k_steps_v2<12, true, false, V2_WIDE_TILE>;
+        if (!global || !v2_tile_ok(1u << logn, global, tile)) return nullptr;
+        if (logn == 12) return bare ? k_steps_v2<12, true, true, V2_WIDE_TILE> : k_steps_v2<12, true, false, V2_WIDE_TILE>;
+        if (logn == 11) return pick_half_v2<11>(bare);
+        if (logn == 10) return pick_half_v2<10>(bare);
+        if (logn == 9) return pick_half_v2<9>(bare);
         return nullptr;
     }
     switch (logn) {
@@ -719,8 +737,11 @@ int set_lds_limit(Ctx* C) {
         }
         dyn_max = std::min(dyn_max, dyn);
     }
-    for (uint32_t i = 0; i < 26; i++) {
-        StepKernelV2 k = i < 24 ? pick_step_v2(9 + i / 4, (i >> 1) & 1, i & 1) : pick_step_v2(12, true, i & 1, V2_WIDE_TILE);
+    for (uint32_t i = 0; i < 32; i++) {
+        // Every length and mode with the default tile, then the other tiles in global mode.
+        const uint32_t logn = i < 24 ? 9 + i / 4 : 9 + (i - 24) / 2;
+        StepKernelV2 k = i < 24 ? pick_step_v2(logn, (i >> 1) & 1, i & 1)
+                                : pick_step_v2(logn, true, i & 1, logn == 12 ? V2_WIDE_TILE : (1u << logn) / 2);
         if (!k) continue;
         hipFuncAttributes a;
         HIPCHK(hipFuncGetAttributes(&a, (const void*)k));
@@ -975,7 +996,7 @@ void set_plan(State* S, const PlanChoice& p) {
     const uint32_t n = S->n, k = S->k;
     S->v2 = p.v2;
     S->global = p.global;
-    S->tile = p.global ? p.tile : 1;
+    S->tile = p.global ? (p.v2 ? v2_tile_norm(S->n, p.tile) : p.tile) : 1;
     S->slots_per_cu = p.slots_per_cu;
     S->v1_tile = 0;
     if (p.v2) {
@@ -994,6 +1015,7 @@ void set_plan(State* S, bool v2, bool global) {
     PlanChoice p;
     p.v2 = v2;
     p.global = global;
+    p.tile = v2 ? v2_default_tile(S->n, global) : 1;
     set_plan(S, p);
 }
 
@@ -1004,11 +1026,12 @@ bool plan_fits(const State* S, const PlanChoice& p) {
     const Ctx* C = S->C;
     if (p.v2 && !(S->wrap && S->n >= 512 && S->n <= 16384)) return false;
     if (p.v2 && S->n > 8192 && !p.global) return false;  // n = 2^14: global mode only
-    if (p.v2 && !v2_tile_ok(S->n, p.global, p.tile)) return false;
+    const uint32_t tile = p.v2 ? v2_tile_norm(S->n, p.tile) : p.tile;
+    if (p.v2 && !v2_tile_ok(S->n, p.global, tile)) return false;
     if (!p.v2 && p.tile != 1 && !(p.global && p.tile == V1_TILE)) return false;
     uint64_t lds;
     if (p.v2) {
-        const uint64_t want = p.tile != 1 ? p.tile : (S->n > 4096 ? 4096 : 2048);
+        const uint64_t want = tile != 1 ? tile : (S->n > 4096 ? 4096 : 2048);
         lds = p.global ? 24 * std::min<uint64_t>(S->n, want) + lds_scratch_bytes(std::min<uint32_t>(S->n / 8, 1024))
                        : v2::v2_lds_resident(S->n, S->k, S->n / 8);
     } else {
@@ -1022,6 +1045,7 @@ bool plan_fits(const State* S, bool v2, bool global) {
     PlanChoice p;
     p.v2 = v2;
     p.global = global;
+    p.tile = v2 ? v2_default_tile(S->n, global) : 1;
     return plan_fits(S, p);
 }
 
@@ -1054,14 +1078,17 @@ int autotune(State* S, bool v2ok) {
         std::vector<PlanChoice> cands;
         for (int v = v2ok ? 1 : 0; v >= 0; v--)
             for (int g = 1; g >= 0; g--)
-                for (uint32_t tile : {1u, V2_WIDE_TILE, V1_TILE})
+                for (uint32_t tile : {1u, V2_WIDE_TILE, V1_TILE, S->n / 2})
                     for (uint32_t spc : {0u, 1u}) {
                         PlanChoice p;
                         p.v2 = v;
                         p.global = g;
-                        p.tile = tile;
+                        p.tile = v && g ? v2_tile_norm(S->n, tile) : tile;
                         p.slots_per_cu = spc;
-                        if (plan_fits(S, p)) cands.push_back(p);
+                        bool seen = false;
+                        for (const PlanChoice& q : cands)
+                            seen |= q.v2 == p.v2 && q.global == p.global && q.tile == p.tile && q.slots_per_cu == p.slots_per_cu;
+                        if (!seen && plan_fits(S, p)) cands.push_back(p);
                     }
         for (const PlanChoice& p : cands) {
             set_plan(S, p);
@@ -1120,8 +1147,7 @@ int mix_class(State* S, uint32_t nt, MixClass* out, uint64_t* lds, uint32_t* gro
         out->kind = ilog2(n);
         out->v = S->vargs();
         *groups = mix_groups(nt, out->kind);
-        *lds = *groups > 1 ? (uint64_t)*groups * mix_group_lds(out->kind)
-                           : 24ull * std::min<uint32_t>(n, n > 4096 ? 4096 : 2048) + lds_scratch_bytes(nt);  // v2::Tiling
+        *lds = *groups > 1 ? (uint64_t)*groups * mix_group_lds(out->kind) : 24ull * mix_tile(out->kind) + lds_scratch_bytes(nt);
         return 0;
     }
     if (S->wrap && n <= 256 && (uint64_t)(nt / 64) * V1_GROUP_LDS <= mix_lds_cap(nt)) {
@@ -1384,7 +1410,7 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         PlanChoice p;
         p.v2 = S->v2;
         p.global = S->global;
-        p.tile = C->opts.tile != 0 ? C->opts.tile : 1;
+        p.tile = C->opts.tile != 0 ? C->opts.tile : S->v2 ? v2_default_tile(n, S->global) : 1;
         p.slots_per_cu = C->opts.slots_per_cu;
         if (p.slots_per_cu > 1 || !plan_fits(S.get(), p))
             return fail("tile " + std::to_string(C->opts.tile) + " / slots per CU " + std::to_string(C->opts.slots_per_cu) +

@@ -18,7 +18,7 @@ __device__ __forceinline__ void mix_v2(const v2::VArgs& A, const uint32_t* list,
     else if constexpr (G > 1)
         steps_v2_group<LOGN, G>(A, list, it.first, it.count, it.steps, lds);
     else
-        steps_v2<LOGN, true, false, ((1u << LOGN) / 8 < NT)>(A, list[it.first], it.steps, lds);
+        steps_v2<LOGN, true, false, ((1u << LOGN) / 8 < NT), mix_tiled(LOGN)>(A, list[it.first], it.steps, lds);
 }
 
 template <uint32_t NT>

@@ -154,11 +154,18 @@ struct MixItem {
     uint32_t cls, first, count, steps;  // slots list[first .. first + count), up to `steps` steps each
 };
 
+// The register-transform step's exchange tile in the mixed launch, as the TILED argument of its
+// templates (v2::Tiling) and in elements: half the vector up to n = 2,048, as in the plain
+// kernel's default plan there (twice the slot groups per workgroup in the same LDS), else the
+// built-in tile.
+__host__ __device__ constexpr uint32_t mix_tiled(uint32_t logn) { return logn <= 11 ? (1u << logn) / 2 : 1u; }
+__host__ __device__ constexpr uint32_t mix_tile(uint32_t logn) {
+    return logn <= 11 ? (1u << logn) / 2 : logn > 12 ? 4096u : 2048u;
+}
 // LDS of one slot group of the register-transform step at log2 n = LOGN in global mode: the
 // exchange tile (24 bytes per element) and the carry scratch of its n / 8 threads, 16-aligned.
 __host__ __device__ constexpr uint32_t mix_group_lds(uint32_t logn) {
-    return ((24u * ((1u << logn) > (logn > 12 ? 4096u : 2048u) ? (logn > 12 ? 4096u : 2048u) : (1u << logn)) +
-             8u * (2u * (((1u << logn) / 8) / 64) + 2u)) + 15u) & ~15u;
+    return ((24u * mix_tile(logn) + 8u * (2u * (((1u << logn) / 8) / 64) + 2u)) + 15u) & ~15u;
 }
 // Slots per workgroup of NT threads at log2 n = LOGN: a power of two, at most NT / (n / 8), and
 // within the LDS of one workgroup per CU (NT = 1,024) or of two (NT = 512) on a 160 KiB CU.
@@ -177,7 +184,8 @@ __host__ __device__ constexpr uint32_t mix_groups(uint32_t nt, uint32_t logn) {
 template <uint32_t LOGN, uint32_t G>
 __device__ __forceinline__ void steps_v2_group(const v2::VArgs& A, const uint32_t* list, uint32_t first, uint32_t count,
                                                uint32_t steps, unsigned char* lds) {
-    constexpr uint32_t T = (1u << LOGN) / 8, tile = v2::Tiling<LOGN, true>::tile;
+    constexpr uint32_t T = (1u << LOGN) / 8, TILED = mix_tiled(LOGN), tile = v2::Tiling<LOGN, TILED>::tile;
+    static_assert(tile == mix_tile(LOGN), "the group's tile is the one its LDS was sized for");
     const Slots& S = A.s;
     const uint32_t n = 1u << LOGN, k = S.k;
     const uint32_t g = __builtin_amdgcn_readfirstlane(threadIdx.x / T);  // whole waves per group
@@ -200,8 +208,8 @@ __device__ __forceinline__ void steps_v2_group(const v2::VArgs& A, const uint32_
         const uint32_t bit = act ? (ex[pos >> 5] >> (31 - (pos & 31))) & 1 : 0;
         const F1* iw1 = bit ? A.iwc1 : A.t.iw1;
         const v2::S2* iw2 = bit ? A.iwc2 : A.t.iw2;
-        bad |= v2::v2_square<LOGN, true, true, T>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv, act);
-        bad |= v2::v2_redc<LOGN, true, true, T>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
+        bad |= v2::v2_square<LOGN, TILED, true, T>(A.t, X, Tl, sc, x, iw1, iw2, A.crt_inv, act);
+        bad |= v2::v2_redc<LOGN, TILED, true, T>(A, X, Tl, sc, x, S.nphat1 + (size_t)slot * n, S.nphat2 + (size_t)slot * n,
                                                   S.nhat1 + (size_t)slot * n, S.nhat2 + (size_t)slot * n, act);
         if (act) pos++;
     }

@@ -2,7 +2,7 @@
 the same moduli run as plain per-length batches, one length after another.
 
     python -m montmul_amd.mixed [--count 1000] [--seed 11] [--min-bits 2048] [--max-bits 200000] \
-        [--ebits 4000] [--check 16]
+        [--ebits 4000] [--check 16] [--parts]
 
 Moduli: `count` random odd numbers whose bit sizes are log-uniform in [min-bits, max-bits]
 (Python `random` seeded `seed`), each at the transform length of its own size (L = 256 ..
@@ -15,6 +15,12 @@ both forms, and read == pow(3, e, N) (GMP) on `check` slots spread over the leng
 
 Prints one JSON line: squarings/s of both forms (squarings = sum of exponent bits), the mixed
 launch's kernel milliseconds (HIP events), slots per length, and the check counts.
+
+`--parts` also times the mixed launch on parts of the workload, to show what bounds it: the
+longest length's slots (plus one shortest-length slot of 16 steps, so that they run in the mixed
+kernel rather than their own), every other slot alone, and every other slot plus one slot of the
+longest length with a 16-bit exponent (which puts the others in that length's workgroup size
+while adding almost no work). Their kernel milliseconds go into the line as "parts_ms".
 """
 
 import argparse
@@ -57,6 +63,7 @@ def main():
     ap.add_argument("--ebits", type=int, default=4000)
     ap.add_argument("--check", type=int, default=16)
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--parts", action="store_true", help="also time the longest length and the rest apart")
     args = ap.parse_args()
     moduli, exps = workload(args.count, args.seed, args.min_bits, args.max_bits, args.ebits)
     squarings = sum(e.bit_length() for e in exps)
@@ -126,12 +133,46 @@ def main():
         "raw_mismatch": raw_mismatch, "powm_checked": len(sample), "powm_mismatch": powm_mismatch,
         "device": hip.device_name(),
     }
+    if args.parts:
+        line["parts_ms"] = parts(hip, moduli, exps, by_len, args.ebits)
     print(json.dumps(line), flush=True)
     for b in plain.values():
         b.free()
     mix.free()
     hip.close()
     sys.exit(0 if raw_mismatch == 0 and powm_mismatch == 0 else 1)
+
+
+def parts(hip, moduli, exps, by_len, ebits):
+    """Kernel milliseconds of the mixed launch on parts of the workload (one untimed pass each)."""
+    longest, shortest = max(by_len), min(by_len)
+    big = by_len[longest]
+    rest = [i for length, idx in sorted(by_len.items()) if length != longest for i in idx]
+    one, tiny = big[0], by_len[shortest][0]
+
+    def short(i):  # the top 16 bits of slot i's exponent: 16 steps
+        return exps[i] >> max(0, exps[i].bit_length() - 16)
+
+    # One length alone would run its own kernel, so the longest length gets one slot of the
+    # shortest beside it, which keeps it in the mixed launch at the cost of 16 short steps.
+    cases = {
+        f"L={longest} ({len(big)} slots) + one L={shortest} slot of 16 steps": (
+            [moduli[i] for i in big] + [moduli[tiny]], [exps[i] for i in big] + [short(tiny)]),
+        f"the rest alone ({len(rest)} slots)": ([moduli[i] for i in rest], [exps[i] for i in rest]),
+        f"the rest + one L={longest} slot of 16 steps": (
+            [moduli[i] for i in rest] + [moduli[one]], [exps[i] for i in rest] + [short(one)]),
+    }
+    out = {}
+    for label, (ms, es) in cases.items():
+        mix = mm.MixBatch(hip, ms, 3)
+        arm(mix, es)
+        mix.run(ebits)
+        arm(mix, es)
+        mix.run(ebits)
+        out[label] = round(mm.Hip.last_run_kernel_ms(_State(mix.group_state(0)))[0], 1)
+        print(f"part: {label}: kernel {out[label]} ms", file=sys.stderr, flush=True)
+        mix.free()
+    return out
 
 
 class _State:

@@ -331,6 +331,27 @@ def test_tile_and_slots_per_cu_plans_match_default():
         h.close()
 
 
+@pytest.mark.parametrize("bits, n", [(9000, 512), (20_000, 1024), (40_000, 2048)])
+def test_half_vector_tile_matches_whole_vector_tile(bits, n):
+    """At L = 1,024 .. 4,096 the register kernel's default exchange tile in global mode is half
+    the vector (two chunks per exchange, T kept in registers); the whole-vector tile, asked for
+    by its size, is the other plan built there. Both give the same raw limbs on every slot."""
+    say("tile", n // 2, "against", n, "bits", bits)
+    whole = Hip(tile=n)
+    try:
+        b = ModBatch(Hip(), [common.rand_odd(random.Random(n), bits)], 3)
+        p = Hip.plan_of(b)
+        assert p["global"] and p["kernel"] == 8 and p["tile"] == n // 2, p
+        b.free()
+        b = ModBatch(whole, [common.rand_odd(random.Random(n), bits)], 3)
+        p = Hip.plan_of(b)
+        assert p["tile"] == n and p["lds_bytes"] > 24 * n, p
+        b.free()
+        common.assert_same_raw(lambda: whole, lambda: Hip(), bits, 64, 96, seed=n)
+    finally:
+        whole.close()
+
+
 def test_two_batches_interleaved(hip):
     """Two live batches with different L, run alternately (tests/gpu.rs:227-258)."""
     rng = random.Random(13)
