@@ -555,12 +555,13 @@ __device__ __forceinline__ void dif_regs(u32 t, Z1* v1, Z2* v2, const F1* tw1, c
             // The two fields' butterflies are independent. Up to n = 4,096 there is no fence
             // between them, so the scheduler may interleave the GF(M31) work with the GF(M61)
             // butterfly (+0.3 % at n = 4,096); the 1,024-thread lengths keep it (their A/B was
-            // -0.4 % at n = 8,192).
+            // -0.4 % at n = 8,192). Nor is there one between the GF(M61) butterfly and its
+            // twiddles, whose loads then issue under the butterfly's arithmetic (+1.5 % at
+            // n = 4,096, +1.4 % at 8,192).
             bfly<Op1, true>(v1, r);
             twiddle<Op1>(v1, tw1 + P::tw_off(p), t, r, ls, P::T);
             if constexpr (LOGN > 12) MM_FENCE();
             bfly<Op2, true>(v2, r);
-            MM_FENCE();
             twiddle<Op2>(v2, tw2 + P::tw_off(p), t, r, ls, P::T);
             MM_FENCE();
         }
@@ -584,7 +585,6 @@ __device__ __forceinline__ void dit_regs(u32 t, Z1* v1, Z2* v2, const F1* itw1, 
             bfly<Op1, false>(v1, r);
             if constexpr (LOGN > 12) MM_FENCE();  // as in dif_regs
             twiddle<Op2>(v2, itw2 + P::tw_off(p), t, r, ls, P::T);
-            MM_FENCE();
             bfly<Op2, false>(v2, r);
             MM_FENCE();
         }
