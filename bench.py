@@ -44,7 +44,7 @@ VALU_PROBE = "profiles/valu_issue_probe.md"
 # be read from inside this process; the summaries are re-measured whenever the kernel changes.
 PROFILE = {8192: "profiles/k_steps_v2_global_100kbit_4096slots.md",
            32768: "profiles/k_steps_v2_split_300kbit_1024slots.md"}
-TRAFFIC_PER_SLOT_STEP = {8192: 282_245, 32768: 9_757_633}
+TRAFFIC_PER_SLOT_STEP = {8192: 324_955, 32768: 9_757_633}
 VALU_PER_SLOT_STEP = {8192: 170_223, 32768: 832_892}
 REF_5090 = 1_536_441.0     # reference, RTX 5090, 170 slots (README.md:14)
 
