@@ -95,6 +95,17 @@ typedef struct mm_backend {
     uint32_t min_len, max_len;
     /* Limits::full_k (transform.rs:17): batches use k = L/2. */
     int32_t full_k;
+    /* Keeps on the device, for `count` slots from `first`, the two raw values that stand for 1:
+     * R mod N and R mod N + N, k limbs each (2k limbs per slot, slot after slot), for
+     * is_one_many. NULL when the backend has no device-side test (ModBatch then reads each
+     * slot). No reference counterpart, like is_one_many. */
+    int (*set_ones)(void *state, size_t first, size_t count, const uint32_t *targets);
+    /* ModBatch::is_one (batch.rs:179-183) on the device for `count` slots: out[i] = 1 if slot
+     * slots[i]'s raw value equals one of the two values set_ones kept for it, else 0. One small
+     * copy back for all of them instead of a read per slot (serving retires finished slots
+     * with it). Returns 0, or nonzero (a slot out of range or without kept values). NULL as
+     * set_ones. */
+    int (*is_one_many)(void *state, const uint32_t *slots, size_t count, int32_t *out);
 } mm_backend;
 
 /* ------------------------------------------------------------------------------------ */
@@ -187,6 +198,11 @@ int mm_last_run_kernel_ms(void *state, double *ms, uint32_t *launches);
 int mm_read_product(void *state, size_t slot, uint32_t *out);
 /* The Hip table's failed_slots (see mm_backend). */
 size_t mm_failed_slots(void *state, uint32_t *out, size_t max);
+/* The Hip table's set_ones and is_one_many (see mm_backend): the two values of 1 per slot are
+ * kept beside x (8k bytes per slot); one workgroup per listed slot compares its x with them.
+ * A slot's kept values are dropped when mm_set_modulus replaces its modulus. */
+int mm_set_ones(void *state, size_t first, size_t count, const uint32_t *targets);
+int mm_is_one_many(void *state, const uint32_t *slots, size_t count, int32_t *out);
 /* mm_mul that allocates nothing. No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). Same result and return values as mm_mul; its device scratch (48 n + 12 k bytes
@@ -303,6 +319,10 @@ int mm_modbatch_read(mm_modbatch *b, size_t slot, uint32_t *out);
 /* ModBatch::is_one (batch.rs:179-183): 1 if the raw value is R mod N or R mod N + N, 0 if
  * not, negative on error. */
 int mm_modbatch_is_one(mm_modbatch *b, size_t slot);
+/* is_one for `count` slots at once: out[i] = 1 or 0 for slot slots[i]. On a backend with
+ * is_one_many the test runs on the device; otherwise each slot is read. Returns 0 or nonzero
+ * on error. No reference counterpart (the reference calls is_one per slot, batch.rs:179-183). */
+int mm_modbatch_is_one_many(mm_modbatch *b, const uint32_t *slots, size_t count, int32_t *out);
 /* k limbs of slot i's modulus (ModBatch::modulus, batch.rs:113-115). */
 int mm_modbatch_modulus(mm_modbatch *b, size_t slot, uint32_t *out);
 /* Replaces slot i's modulus by another odd N > 1 needing the batch's L (k limbs at most): N', R

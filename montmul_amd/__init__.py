@@ -63,6 +63,8 @@ _SETMOD = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.
 _RUNGROUP = C.CFUNCTYPE(C.c_int64, C.POINTER(C.c_void_p), C.c_size_t, C.c_uint64, C.POINTER(C.c_size_t))
 _FAILED = C.CFUNCTYPE(C.c_size_t, C.c_void_p, C.POINTER(C.c_uint32), C.c_size_t)
 _LASTERR = C.CFUNCTYPE(C.c_char_p)
+_SETONES = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.c_size_t, C.POINTER(C.c_uint32))
+_ISONEMANY = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_int32))
 
 
 class BackendTable(C.Structure):
@@ -72,7 +74,8 @@ class BackendTable(C.Structure):
                 ("remaining", _REMAINING), ("set_modulus", _SETMOD), ("run_group", _RUNGROUP),
                 ("failed_slots", _FAILED), ("last_error", _LASTERR),
                 ("bound_lo", C.c_uint64), ("bound_hi", C.c_uint64),
-                ("min_len", C.c_uint32), ("max_len", C.c_uint32), ("full_k", C.c_int32)]
+                ("min_len", C.c_uint32), ("max_len", C.c_uint32), ("full_k", C.c_int32),
+                ("set_ones", _SETONES), ("is_one_many", _ISONEMANY)]
 
 
 _OPEN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(BackendTable))
@@ -159,6 +162,7 @@ def _load_lib():
         "mm_modbatch_read_raw": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_modbatch_read": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_modbatch_is_one": (C.c_int, [vp, C.c_size_t]),
+        "mm_modbatch_is_one_many": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_int32)]),
         "mm_modbatch_modulus": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_modbatch_set_modulus": (C.c_int, [vp, C.c_size_t, C.c_int, u32p, C.c_size_t]),
         "mm_set_modulus": (C.c_int, [vp, C.c_size_t, u32p, u32p, u32p]),
@@ -671,6 +675,15 @@ class ModBatch:
         if r < 0:
             raise MontmulError(_host_error())
         return bool(r)
+
+    def is_one_many(self, slots):
+        """is_one for many slots at once (on the device on the Hip backend): a list of bools."""
+        slots = list(slots)
+        arr = (C.c_uint32 * max(1, len(slots)))(*slots)
+        out = (C.c_int32 * max(1, len(slots)))()
+        if lib.mm_modbatch_is_one_many(self._h, arr, len(slots), out):
+            raise MontmulError(_host_error())
+        return [bool(out[j]) for j in range(len(slots))]
 
 
 class MixBatch:

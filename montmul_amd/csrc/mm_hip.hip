@@ -51,6 +51,25 @@ __global__ __launch_bounds__(LOGN > 12 ? 1024 : 512) __attribute__((amdgpu_waves
     steps_v2<LOGN, GLOBAL, BARE, false, TILE, true>(A, first + blockIdx.x, bits, lds);
 }
 
+// ModBatch::is_one (batch.rs:179-183) for a list of slots, one workgroup per listed slot:
+// out[b] = 1 if slot list[b]'s raw value (k limbs of x) equals either of its two kept values
+// (R mod N, then R mod N + N: 2k limbs per slot in `ones`), else 0.
+__global__ void k_is_one(const uint32_t* __restrict__ x, const uint32_t* __restrict__ ones, const uint32_t* __restrict__ list,
+                         uint32_t k, int32_t* __restrict__ out) {
+    const uint32_t slot = list[blockIdx.x];
+    const uint32_t* xs = x + (size_t)slot * k;
+    const uint32_t* a = ones + (size_t)slot * 2 * k;
+    bool ne0 = false, ne1 = false;
+    for (uint32_t j = threadIdx.x; j < k; j += blockDim.x) {
+        const uint32_t v = xs[j];
+        ne0 |= v != a[j];
+        ne1 |= v != a[k + j];
+    }
+    const int d0 = __syncthreads_or(ne0);
+    const int d1 = __syncthreads_or(ne1);
+    if (threadIdx.x == 0) out[blockIdx.x] = (!d0 || !d1) ? 1 : 0;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t count, uint32_t v) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < count) p[j] = v;
@@ -597,6 +616,12 @@ struct State {
     uint32_t* d_stage_lim = nullptr;
     F1* d_stage1 = nullptr;
     F2* d_stage2 = nullptr;
+    // is_one on the device: the two raw values that stand for 1 (2k limbs per slot), a slot list
+    // and a flag per listed slot.
+    uint32_t* d_ones = nullptr;
+    uint32_t* d_ones_list = nullptr;
+    int32_t* d_ones_out = nullptr;
+    std::vector<uint8_t> ones_set;
     uint32_t ew = 0;                  // exponent row stride in words
     size_t exp_cap = 0;               // words allocated for the exponent rows
     // host mirrors
@@ -619,7 +644,8 @@ struct State {
         hipSetDevice(device);
         for (void* p : {(void*)d_x, (void*)d_exp, (void*)d_nbits, (void*)d_pos, (void*)d_err, (void*)d_lim,
                         (void*)d_nphat1, (void*)d_nhat1, (void*)d_work1, (void*)d_nphat2, (void*)d_nhat2, (void*)d_work2,
-                        (void*)d_iwc1, (void*)d_iwc2, (void*)d_stage_lim, (void*)d_stage1, (void*)d_stage2, (void*)d_coef})
+                        (void*)d_iwc1, (void*)d_iwc2, (void*)d_stage_lim, (void*)d_stage1, (void*)d_stage2, (void*)d_coef,
+                        (void*)d_ones, (void*)d_ones_list, (void*)d_ones_out})
             if (p) hipFree(p);
     }
     v2::VArgs vargs() const {
@@ -1355,6 +1381,8 @@ int mm_hip_open(const mm_options* opts, mm_backend* out) {
     out->remaining = mm_remaining;
     out->failed_slots = mm_failed_slots;
     out->last_error = mm_last_error;
+    out->set_ones = mm_set_ones;
+    out->is_one_many = mm_is_one_many;
     // Limits (gpu/mod.rs:272-283): bound Q = P1 * P2, lengths 2^3..2^24, full_k = wraparound.
     u128 Q = (u128)P1 * P2;
     out->bound_lo = (uint64_t)Q;
@@ -1418,9 +1446,10 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
         set_plan(S.get(), p);
     }
     // Device memory: x, N'-hat, N-hat or A|B, work vectors and limbs in global mode, exponents
-    // of up to k words, per-slot words, tables (about 6n elements per field, twice), a margin.
+    // of up to k words, per-slot words, the two raw values of 1 with a list word and a flag,
+    // tables (about 6n elements per field, twice), a margin.
     const uint64_t per_slot = 4ull * k + 48ull * n + (S->global || tune ? 24ull * n + 12ull * k : 0) + 16 + 4ull * k +
-                              (v2ok && n > 8192 ? 24ull * n : 0);
+                              (v2ok && n > 8192 ? 24ull * n : 0) + 8ull * k + 8;
     const uint64_t need = (uint64_t)st->slots * per_slot + 300ull * n + 48ull * n * 2 + (64ull << 20);
     size_t freeb = 0, total = 0;
     if (hipMemGetInfo(&freeb, &total) != hipSuccess) return fail("hipMemGetInfo failed");
@@ -1428,6 +1457,8 @@ int mm_create(void* self, const mm_batch_setup* st, void** out) {
     if (get_tables(C, n, &S->tab)) return 1;
     if (wrap && get_tables(C, n / 2, &S->half)) return 1;
     const size_t sl = st->slots;
+    if (dalloc(&S->d_ones, sl * 2 * k) || dalloc(&S->d_ones_list, sl) || dalloc(&S->d_ones_out, sl)) return 1;
+    S->ones_set.assign(sl, 0);
     if (dalloc(&S->d_x, sl * k) || dalloc(&S->d_nbits, sl) || dalloc(&S->d_pos, sl) || dalloc(&S->d_err, sl) ||
         dalloc(&S->d_nphat1, sl * n) || dalloc(&S->d_nphat2, sl * n) || dalloc(&S->d_nhat1, sl * n) ||
         dalloc(&S->d_nhat2, sl * n) || dalloc(&S->d_exp, sl))
@@ -1528,6 +1559,7 @@ int mm_set_modulus(void* st, size_t i, const uint32_t* nl, const uint32_t* npl, 
     (void)two_n;  // the device path has no use for 2N
     if (i >= S->slots) return fail("slot out of range");
     if (hipSetDevice(C->device) != hipSuccess) return fail("hipSetDevice failed");
+    S->ones_set[i] = 0;  // the values of 1 belong to the old modulus until set_ones
     const uint32_t k = S->k, n = S->n;
     hipStream_t sm = C->stream;
     if (!S->d_stage_lim && dalloc(&S->d_stage_lim, 2ull * k)) return 1;
@@ -1648,6 +1680,45 @@ int64_t mm_mul_noalloc(void* st, size_t out, size_t a, size_t b) {
         return (int64_t)out;
     }
     return MM_OK;
+}
+
+// The raw values that stand for 1, for is_one_many (no reference counterpart).
+int mm_set_ones(void* st, size_t first, size_t count, const uint32_t* targets) {
+    State* S = (State*)st;
+    if (first > S->slots || count > S->slots - first) return fail("set_ones: slot range past the batch");
+    if (count == 0) return 0;
+    if (hipSetDevice(S->C->device) != hipSuccess) return fail("hipSetDevice failed");
+    const size_t w = 2ull * S->k;
+    HIPCHK(hipMemcpyAsync(S->d_ones + first * w, targets, count * w * 4, hipMemcpyHostToDevice, S->C->stream));
+    HIPCHK(hipStreamSynchronize(S->C->stream));
+    std::fill(S->ones_set.begin() + first, S->ones_set.begin() + first + count, 1);
+    return 0;
+}
+
+// ModBatch::is_one (batch.rs:179-183) for many slots on the device: the rows written on the host
+// go up first, then one workgroup compares each listed slot's x with its two kept values, and
+// one copy brings the flags back.
+int mm_is_one_many(void* st, const uint32_t* slots, size_t count, int32_t* out) {
+    State* S = (State*)st;
+    for (size_t i = 0; i < count; i++) {
+        if (slots[i] >= S->slots) return fail("is_one_many: slot " + std::to_string(slots[i]) + " out of range");
+        if (!S->ones_set[slots[i]]) return fail("is_one_many: slot " + std::to_string(slots[i]) + " has no value of 1 kept");
+    }
+    if (count == 0) return 0;
+    if (hipSetDevice(S->C->device) != hipSuccess) return fail("hipSetDevice failed");
+    if (push_x(S)) return 1;
+    hipStream_t sm = S->C->stream;
+    // The list and flag buffers hold one word per slot of the batch: longer lists (a slot may
+    // be listed more than once) go in parts of that size.
+    for (size_t a = 0; a < count; a += S->slots) {
+        const size_t c = std::min<size_t>(S->slots, count - a);
+        HIPCHK(hipMemcpyAsync(S->d_ones_list, slots + a, c * 4, hipMemcpyHostToDevice, sm));
+        hipLaunchKernelGGL(k_is_one, dim3((uint32_t)c), dim3(256), 0, sm, S->d_x, S->d_ones, S->d_ones_list, S->k, S->d_ones_out);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out + a, S->d_ones_out, c * 4, hipMemcpyDeviceToHost, sm));
+        HIPCHK(hipStreamSynchronize(sm));
+    }
+    return 0;
 }
 
 int mm_read(void* st, size_t i, uint32_t* out) {  // gpu/mod.rs:392-394

@@ -117,6 +117,36 @@ struct mm_modbatch {  // batch.rs:53-58
     ~mm_modbatch() { if (state) be.destroy(state); }
 };
 
+namespace {
+
+// The two raw values that stand for 1 in slot i (ModBatch::is_one, batch.rs:179-183): R mod N
+// and R mod N + N, k limbs each, into out[0 .. 2k).
+bool ones_of(const MmModulus& m, uint32_t k, uint32_t* out) {
+    Mpz r2;
+    mpz_add(r2.v, m.r.v, m.n.v);
+    return to_limbs(m.r.v, out, k) && to_limbs(r2.v, out + k, k);
+}
+
+// Hands slots [first, first + count)'s values of 1 to a backend that tests is_one on the device,
+// 256 slots per call.
+int keep_ones(mm_modbatch* b, size_t first, size_t count) {
+    if (!b->be.set_ones) return 0;
+    const size_t w = 2ull * b->k;
+    for (size_t a = first; a < first + count; a += 256) {
+        const size_t c = std::min<size_t>(256, first + count - a);
+        std::vector<uint32_t> t(c * w);
+        std::atomic<int> bad(0);
+        par_for(c, [&](size_t j) {
+            if (!ones_of(b->moduli[a + j], b->k, &t[j * w])) bad = 1;
+        });
+        if (bad) return fail("R mod N + N does not fit " + std::to_string(b->k) + " limbs");
+        if (b->be.set_ones(b->state, a, c, t.data())) return fail(b->be.last_error());
+    }
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* mm_host_last_error(void) { return g_err.c_str(); }
@@ -202,6 +232,10 @@ int mm_modbatch_new(const mm_backend* be, size_t count, const uint32_t* limbs, c
         delete B;
         return 1;
     }
+    if (keep_ones(B, 0, count)) {
+        delete B;
+        return 1;
+    }
     *out = B;
     return 0;
 }
@@ -232,7 +266,7 @@ int mm_modbatch_set_modulus(mm_modbatch* b, size_t i, int sign, const uint32_t* 
     mpz_set(b->moduli[i].n.v, m.n.v);
     mpz_set(b->moduli[i].r.v, m.r.v);
     mpz_set(b->moduli[i].rinv.v, m.rinv.v);
-    return 0;
+    return keep_ones(b, i, 1);
 }
 
 void mm_modbatch_free(mm_modbatch* b) { delete b; }
@@ -346,6 +380,18 @@ int mm_modbatch_is_one(mm_modbatch* b, size_t i) {  // batch.rs:179-183
     mpz_import(v.v, b->k, -1, 4, 0, 0, l.data());
     mpz_add(r2.v, m.r.v, m.n.v);
     return mpz_cmp(v.v, m.r.v) == 0 || mpz_cmp(v.v, r2.v) == 0;
+}
+
+int mm_modbatch_is_one_many(mm_modbatch* b, const uint32_t* slots, size_t count, int32_t* out) {
+    for (size_t j = 0; j < count; j++)
+        if (check_slot(b, slots[j])) return 1;
+    if (b->be.is_one_many) return b->be.is_one_many(b->state, slots, count, out) ? fail(b->be.last_error()) : 0;
+    for (size_t j = 0; j < count; j++) {
+        const int v = mm_modbatch_is_one(b, slots[j]);
+        if (v < 0) return 1;
+        out[j] = v;
+    }
+    return 0;
 }
 
 int mm_modbatch_modulus(mm_modbatch* b, size_t i, uint32_t* out) {
@@ -556,11 +602,11 @@ int mm_fermat(const mm_backend* be, size_t count, const uint32_t* limbs, const s
         int64_t r = mm_modbatch_run(b, chunk);
         if (r != MM_OK) { mm_modbatch_free(b); return 1; }
     }
-    for (size_t i = 0; i < count; i++) {
-        int v = mm_modbatch_is_one(b, i);
-        if (v < 0) { mm_modbatch_free(b); return 1; }
-        prp[i] = (uint8_t)v;
-    }
+    std::vector<uint32_t> all(count);
+    std::vector<int32_t> ones(count);
+    for (size_t i = 0; i < count; i++) all[i] = (uint32_t)i;
+    if (mm_modbatch_is_one_many(b, all.data(), count, ones.data())) { mm_modbatch_free(b); return 1; }
+    for (size_t i = 0; i < count; i++) prp[i] = (uint8_t)ones[i];
     mm_modbatch_free(b);
     return 0;
 }

@@ -7,8 +7,9 @@
 //   1. admit waiting requests into free slots (mm_modbatch_set_modulus, load 1, exponent N - 1);
 //   2. run every slot with bits left for a chunk of steps sized to `target_ms` from the pool's
 //      measured time per step (slots with no bits left are skipped by the kernel);
-//   3. retire finished slots: is_one on the slot's raw value (a read of that slot's k limbs
-//      only) and a completion record.
+//   3. retire finished slots: is_one on their raw values, all at once (on the device when the
+//      backend tests it there: one kernel and a copy of one word per slot) and a completion
+//      record each.
 // Pools run concurrently, so requests of different lengths share the GPU through streams.
 //
 // Admission by device memory: with hbm_bytes set, a pool reserves slots x mm_server_slot_bytes(L)
@@ -317,14 +318,21 @@ struct Pool {
                     if (slot_req[i] >= 0 && bad[i]) finish(i, 0, why.c_str());
                 continue;
             }
-            for (size_t i = 0; i < slot_req.size(); i++) {
-                if (slot_req[i] < 0 || mm_modbatch_remaining(batch, i) > 0) continue;
-                const int one = mm_modbatch_is_one(batch, i);
+            // Finished slots: is_one for all of them at once (on the device, one small copy).
+            std::vector<uint32_t> fin;
+            for (size_t i = 0; i < slot_req.size(); i++)
+                if (slot_req[i] >= 0 && mm_modbatch_remaining(batch, i) == 0) fin.push_back((uint32_t)i);
+            if (fin.empty()) continue;
+            std::vector<int32_t> one(fin.size(), -1);
+            const bool ok = mm_modbatch_is_one_many(batch, fin.data(), fin.size(), one.data()) == 0;
+            const std::string why = ok ? std::string() : "is_one failed: " + std::string(mm_host_last_error());
+            for (size_t j = 0; j < fin.size(); j++) {
+                const size_t i = fin[j];
                 {
                     std::lock_guard<std::mutex> g(S->mu);
                     S->stats.squarings += slot_bits[i] ? slot_bits[i] - 1 : 0;
                 }
-                finish(i, one, one < 0 ? "read failed" : nullptr);
+                finish(i, ok ? one[j] : -1, ok ? nullptr : why.c_str());
             }
         }
         if (batch) mm_modbatch_free(batch);
@@ -358,7 +366,7 @@ void hip_close(void* user, mm_backend* be) {
 extern "C" {
 
 // At L = 32,768 the register-transform step also stages its coefficients in HBM (24 n = 12 L).
-uint64_t mm_server_slot_bytes(uint32_t len) { return 46ull * len + 16 + (len == 32768 ? 12ull * len : 0); }
+uint64_t mm_server_slot_bytes(uint32_t len) { return 50ull * len + 24 + (len == 32768 ? 12ull * len : 0); }
 
 int mm_server_new(const mm_server_options* o, mm_open_fn open, mm_close_fn close, void* user, mm_server** out) {
     *out = nullptr;

@@ -480,6 +480,46 @@ def set_modulus_admission(make, bits, seed=8):
         raise AssertionError("a modulus of another transform length was accepted")
 
 
+def is_one_many_check(make, bits, seed=12, prime=None):
+    """ModBatch.is_one_many (is_one of batch.rs:179-183 for many slots at once) agrees with
+    is_one and with the definition on every representative that matters: R mod N and
+    R mod N + N (1), R mod N + 1, 0 and 2N - 1 (not 1), for slots in any order and repeated.
+    With `prime` (a known prime of the batch's transform length), also after Fermat runs on
+    composites and on that prime, admitted into a slot by set_modulus mid-batch (its values of
+    1 follow the new modulus)."""
+    rng = random.Random(seed)
+    moduli = [rand_odd(rng, bits) for _ in range(6)]
+    b = ModBatch(make(), moduli, 3)
+    k = b.params().k
+    reps = []
+    for i, n in enumerate(moduli):
+        r = (1 << (32 * k)) % n
+        v = [r, r + n, r + 1, 0, 2 * n - 1, r][i]
+        b.load_raw(i, v)
+        reps.append(v in (r, r + n))
+    order = [5, 0, 3, 1, 1, 4, 2]
+    got = b.is_one_many(order)
+    assert got == [reps[i] for i in order], (got, reps)
+    assert got == [b.is_one(i) for i in order]
+    if prime is not None:
+        p = prime
+        b.set_modulus(3, p)
+        for i, n in enumerate(moduli[:3] + [p]):
+            b.load(i, 1)
+            b.set_exponent(i, n - 1)
+        b.run(1 << 20)
+        got = b.is_one_many(range(4))
+        assert got == [b.is_one(i) for i in range(4)]
+        assert got[3], "a prime modulus admitted by set_modulus came out composite"
+        assert got[:3] == [powmod(3, n - 1, n) == 1 for n in moduli[:3]]
+    try:
+        b.is_one_many([len(moduli)])
+    except MontmulError:
+        pass
+    else:
+        raise AssertionError("a slot past the batch was accepted")
+
+
 def serve_mix(server, make, moduli, base=3, timeout_s=120.0):
     """Submits every modulus to `server` (a montmul_amd.Server with keep_raw), waits for all,
     and checks each result: prp == [base^(N-1) = 1 mod N] and the final raw limbs equal those of

@@ -352,6 +352,16 @@ def test_half_vector_tile_matches_whole_vector_tile(bits, n):
         whole.close()
 
 
+@pytest.mark.parametrize("bits, mersenne", [(3000, 3217), (20_000, 19937), (100_000, 0)])
+def test_is_one_many_on_the_device(hip_global, bits, mersenne):
+    """is_one for many slots by one kernel and one copy (the Hip backend's is_one_many), against
+    is_one and the definition: radix-4 (L = 256) and register kernels (L = 2,048, 8,192); Fermat
+    runs on composites and on a Mersenne prime of the same length admitted by set_modulus
+    (L = 256, 2,048)."""
+    say("is_one_many", bits)
+    common.is_one_many_check(lambda: hip_global, bits, prime=(1 << mersenne) - 1 if mersenne else None)
+
+
 def test_two_batches_interleaved(hip):
     """Two live batches with different L, run alternately (tests/gpu.rs:227-258)."""
     rng = random.Random(13)

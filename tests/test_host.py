@@ -76,6 +76,12 @@ def test_is_one_accepts_upper_lazy_representative():  # tests/cpu.rs:83-95
     assert not b.is_one(0)
 
 
+@pytest.mark.parametrize("name", ["cpu", "gmp"])
+def test_is_one_many_on_oracle(name):
+    """Backends without a device-side test: is_one_many reads each slot."""
+    common.is_one_many_check(lambda: Cpu() if name == "cpu" else Gmp(), 1000, prime=2**607 - 1)
+
+
 def test_load_raw_rejects_out_of_range():  # tests/cpu.rs:97-103
     n = 1_000_003
     b = ModBatch(Cpu(), [n], 3)
