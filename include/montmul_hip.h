@@ -353,6 +353,8 @@ uint64_t mm_mixbatch_remaining(mm_mixbatch *b, size_t slot);
 int mm_mixbatch_read_raw(mm_mixbatch *b, size_t slot, uint32_t *out);
 int mm_mixbatch_read(mm_mixbatch *b, size_t slot, uint32_t *out);
 int mm_mixbatch_is_one(mm_mixbatch *b, size_t slot);
+/* mm_modbatch_is_one_many over a mixed batch: each length's listed slots are tested together. */
+int mm_mixbatch_is_one_many(mm_mixbatch *b, const uint32_t *slots, size_t count, int32_t *out);
 int mm_mixbatch_check_pow(mm_mixbatch *b, size_t slot, uint32_t base, const uint32_t *e, size_t ne);
 /* The backend state of group `group` (groups in ascending L), NULL if out of range. */
 void *mm_mixbatch_group_state(mm_mixbatch *b, size_t group);

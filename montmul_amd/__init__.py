@@ -195,6 +195,7 @@ def _load_lib():
         "mm_mixbatch_read_raw": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_mixbatch_read": (C.c_int, [vp, C.c_size_t, u32p]),
         "mm_mixbatch_is_one": (C.c_int, [vp, C.c_size_t]),
+        "mm_mixbatch_is_one_many": (C.c_int, [vp, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_int32)]),
         "mm_mixbatch_check_pow": (C.c_int, [vp, C.c_size_t, C.c_uint32, u32p, C.c_size_t]),
         "mm_mixbatch_group_state": (vp, [vp, C.c_size_t]),
         "mm_t_upload": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, C.POINTER(vp)]),
@@ -787,6 +788,15 @@ class MixBatch:
         if r < 0:
             raise MontmulError(_host_error())
         return bool(r)
+
+    def is_one_many(self, slots):
+        """is_one for many slots at once (each length's slots together, on the device on Hip)."""
+        slots = list(slots)
+        arr = (C.c_uint32 * max(1, len(slots)))(*slots)
+        out = (C.c_int32 * max(1, len(slots)))()
+        if lib.mm_mixbatch_is_one_many(self._h, arr, len(slots), out):
+            raise MontmulError(_host_error())
+        return [bool(out[j]) for j in range(len(slots))]
 
 
 def fermat(backend, moduli, base, chunk):

@@ -564,6 +564,24 @@ int mm_mixbatch_is_one(mm_mixbatch* b, size_t i) {
     return g ? mm_modbatch_is_one(g, j) : -1;
 }
 
+int mm_mixbatch_is_one_many(mm_mixbatch* b, const uint32_t* slots, size_t count, int32_t* out) {
+    // Per length: the listed slots of each group go to that group's is_one_many together.
+    std::vector<std::vector<uint32_t>> idx(b->groups.size()), pos(b->groups.size());
+    for (size_t j = 0; j < count; j++) {
+        if (slots[j] >= b->where.size()) return fail("slot " + std::to_string(slots[j]) + " out of range");
+        const auto& w = b->where[slots[j]];
+        idx[w.first].push_back(w.second);
+        pos[w.first].push_back((uint32_t)j);
+    }
+    for (size_t g = 0; g < b->groups.size(); g++) {
+        if (idx[g].empty()) continue;
+        std::vector<int32_t> r(idx[g].size());
+        if (mm_modbatch_is_one_many(b->groups[g], idx[g].data(), idx[g].size(), r.data())) return 1;
+        for (size_t j = 0; j < r.size(); j++) out[pos[g][j]] = r[j];
+    }
+    return 0;
+}
+
 int mm_mixbatch_check_pow(mm_mixbatch* b, size_t i, uint32_t base, const uint32_t* e, size_t ne) {
     size_t j;
     mm_modbatch* g = mix_slot(b, i, &j);

@@ -520,6 +520,29 @@ def is_one_many_check(make, bits, seed=12, prime=None):
         raise AssertionError("a slot past the batch was accepted")
 
 
+def mix_is_one_many_check(make, seed=13):
+    """MixBatch.is_one_many after full Fermat tests over several lengths: Mersenne primes
+    (2^p - 1, p = 127 .. 4423) and random composites of the same sizes, slots listed out of
+    order and repeated; each answer equals is_one and the primality of the modulus."""
+    rng = random.Random(seed)
+    exps = [127, 521, 1279, 2203, 3217, 4423]
+    moduli = []
+    for p in exps:
+        moduli.append((1 << p) - 1)
+        moduli.append(rand_odd(rng, p))
+    b = MixBatch(make(), moduli, 3)
+    for i, n in enumerate(moduli):
+        b.load(i, 1)
+        b.set_exponent(i, n - 1)
+    b.run(1 << 20)
+    order = list(range(len(moduli)))
+    rng.shuffle(order)
+    order += order[:3]
+    got = b.is_one_many(order)
+    assert got == [b.is_one(i) for i in order]
+    assert got == [i % 2 == 0 for i in order], got  # even slots hold the primes
+
+
 def serve_mix(server, make, moduli, base=3, timeout_s=120.0):
     """Submits every modulus to `server` (a montmul_amd.Server with keep_raw), waits for all,
     and checks each result: prp == [base^(N-1) = 1 mod N] and the final raw limbs equal those of

@@ -140,3 +140,8 @@ def test_mixed_raw_limbs_against_the_closed_form(hip_global):
     sizes = [3000, 6000, 9000, 20000, 40000, 60000, 90000, 120000, 150000, 200000] * 2
     lengths = common.mixed_lengths(lambda: hip_global, sizes, 24, 7, seed=21, ref=lambda: Gmp(full_k=True))
     assert lengths == [256, 512, 1024, 2048, 4096, 8192, 16384]
+
+
+def test_mixed_is_one_many(hip_global):
+    """Fermat tests over six lengths in one mixed batch, retired by is_one_many on the device."""
+    common.mix_is_one_many_check(lambda: hip_global)

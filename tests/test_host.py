@@ -82,6 +82,10 @@ def test_is_one_many_on_oracle(name):
     common.is_one_many_check(lambda: Cpu() if name == "cpu" else Gmp(), 1000, prime=2**607 - 1)
 
 
+def test_mix_is_one_many_on_oracle():
+    common.mix_is_one_many_check(lambda: Gmp(full_k=True))
+
+
 def test_load_raw_rejects_out_of_range():  # tests/cpu.rs:97-103
     n = 1_000_003
     b = ModBatch(Cpu(), [n], 3)
